@@ -1,0 +1,195 @@
+"""Benchmark: J-tensor grid points/s on one MI355X, mixed precision vs FP64.
+
+Workload (one step): the full J tensor on every point of a 128^3 grid
+(0.12 bohr spacing) around a 60-atom carbon cage (truncated icosahedron,
+radius 6.7 bohr) with a 4s3p2d1f spherical basis per atom (nbf 1800),
+nocc 180, seeded synthetic densities.  Points and results stay in HBM.
+`value` is the mixed (split-BF16 MFMA, FP32 accumulation, FP64 assembly)
+rate; the FP64 GPU path, the plane-flux error of mixed against FP64, the
+contraction kernel's roofline position and the reference CPU code timed on
+this host are reported beside it.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"bf16x3": 2500.0, "fp64": 78.6}  # MI355X dense MFMA (bf16, fp64)
+HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def plane_grid(system):
+    """bond plane through the cage: atoms 1 and its nearest neighbour"""
+    from gimic_amd import grid as gridmod
+    c = np.asarray(system.coords)
+    d = np.linalg.norm(c - c[0], axis=1)
+    d[0] = np.inf
+    j = int(np.argmin(d))
+    kw = {"_arg": "bond", "type": "lobatto", "gauss_order": 7, "bond": [1, j + 1],
+          "fixcoord": [0.0, 0.0, 0.0], "distance": float(d[j] / 2), "height": [-7.0, 7.0],
+          "width": [-7.0, 7.0], "grid_points": [140, 140, 0]}
+    g = gridmod.new_grid(kw, c)
+    return g, gridmod.get_magnet(g, "X")
+
+
+def time_steps(ctx, d_pts, out, precision, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        ctx.jtensor_device(d_pts, precision=precision, out=out)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        ctx.jtensor_device(d_pts, precision=precision, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall
+
+
+def kernel_roofline(ctx, d_pts, out, precision, nbf):
+    """one extra pass with event timing around each kernel launch"""
+    import torch
+    ctx.set_timing(True)
+    ctx.jtensor_device(d_pts, precision=precision, out=out)
+    torch.cuda.synchronize()
+    t = ctx.get_timing()
+    ctx.set_timing(False)
+    ms, launches = t["contract"]
+    npts = d_pts.shape[0]
+    flop = 14.0 * nbf * nbf * npts  # algorithmic: 7 products of n x n per point
+    achieved = flop / (ms / launches) / launches / 1e9  # TFLOP/s per launch average
+    avg_ms = ms / launches
+    return dict(bound="mfma", achieved=round(achieved, 2), peak=PEAK_TFLOPS[precision],
+                unit="TFLOP/s", frac=round(achieved / PEAK_TFLOPS[precision], 4), traffic=None,
+                kernel="contract_bf16x3_kernel" if precision != "fp64" else "contract_f64_kernel",
+                avg_launch_ms=round(avg_ms, 3), launches=launches,
+                flop_per_launch=flop / launches), t
+
+
+def cpu_baseline(system, budget_s=15.0):
+    """reference (flang-built oracle/_ref/ref_driver) on a bounded sample"""
+    from oracle import molio, refrun, sph
+    if not refrun.available():
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    b = molio.Basis()
+    b.symbols, b.charges, b.coords = system.symbols, system.charges, system.coords
+    b.shell_atom, b.shell_l, b.shell_nprim, b.shell_poff = (system.shell_atom, system.shell_l,
+                                                             system.shell_nprim, system.shell_poff)
+    b.exps, b.coefs = system.exps, system.coefs
+    dcart = sph.to_cartesian(system.dens, system.shell_l)
+    rng = np.random.default_rng(3)
+    from gimic_amd import synth
+    pts = synth.cube_grid(128, 0.12, -7.62)
+    with tempfile.TemporaryDirectory() as td:
+        mol = os.path.join(td, "MOL")
+        molio.write_mol(mol, b)
+        flat = molio.pack_dens(dcart)
+        sample = pts[rng.choice(len(pts), 64, replace=False)]
+        _, t = refrun.ref_jtensor(mol, sample, dens=flat, screening=1e-8, threads=threads,
+                                  return_time=True)
+        n = int(min(len(pts), max(64, budget_s / max(t, 1e-6) * 64)))
+        n = (n // threads) * threads or threads
+        sample = pts[rng.choice(len(pts), n, replace=False)]
+        _, t = refrun.ref_jtensor(mol, sample, dens=flat, screening=1e-8, threads=threads,
+                                  return_time=True)
+    return {"value": round(n / t, 2), "unit": "points/s", "cores": threads, "kind": "reference",
+            "sample": f"{n} random points of the 128^3 grid, flang-built reference "
+                      f"(oracle/_ref/ref_driver, OpenMP over points), cartesian-equivalent "
+                      f"density D_cart = T^T D T, loop time {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
+    ap.add_argument("--points", type=int, default=0, help="limit points (profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the workload is sharded by point tiles "
+              "across GPUs by design but this build runs on a single MI355X.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from gimic_amd import flux, synth
+    assert torch.cuda.is_available(), "no GPU"
+    dev = torch.device("cuda:0")
+    log("setup cage60")
+    system = synth.cage60()
+    nbf = system.nbf
+    pts = synth.cube_grid(128, 0.12, -7.62)
+    if args.points:
+        pts = pts[:args.points]
+    npts = len(pts)
+    ctx = system.context(screening=1e-8)
+    d_pts = torch.tensor(pts, dtype=torch.float64, device=dev)
+    out = torch.empty((npts, 9), dtype=torch.float64, device=dev)
+
+    log("timing mixed")
+    ms_mixed, wall_mixed = time_steps(ctx, d_pts, out, "bf16x3", args.steps, args.warmup)
+    J_mixed_sample = out[:: max(1, npts // 65536)].cpu().numpy()
+    roof, ktimes = kernel_roofline(ctx, d_pts, out, "bf16x3", nbf)
+    log("timing fp64")
+    ms_f64, wall_f64 = time_steps(ctx, d_pts, out, "fp64", args.steps, args.warmup)
+    J_f64_sample = out[:: max(1, npts // 65536)].cpu().numpy()
+    roof64, _ = kernel_roofline(ctx, d_pts, out, "fp64", nbf)
+
+    nb = np.linalg.norm(J_f64_sample, axis=1)
+    e = np.linalg.norm(J_mixed_sample - J_f64_sample, axis=1) / np.maximum(nb, 1e-3 * nb.max())
+    log("plane flux")
+    g, B = plane_grid(system)
+    f64 = np.array(flux.integrate_current(ctx, g, B, precision="fp64"))
+    fmx = np.array(flux.integrate_current(ctx, g, B, precision="bf16x3"))
+    flux_err = float(abs(fmx[0] - f64[0]) / abs(f64[0]))
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(system)
+
+    value = npts / (ms_mixed / 1e3)
+    res = {
+        "metric": "J-tensor grid points/s, mixed vs FP64 GPU at <=1e-6 rel plane-flux error",
+        "value": round(value, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_mixed, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3-mfma/fp32-acc/fp64-assembly",
+        "data": "synthetic (seeded basis and densities)",
+        "config": {"workload": "cage60 C60 truncated icosahedron r=6.7 bohr, 4s3p2d1f spherical "
+                               "(nbf 1800), nocc 180, 128^3 grid at 0.12 bohr",
+                   "points_per_step": npts, "nbf": nbf, "precision": "mixed split-BF16x3"},
+        "fp64_gpu": {"value": round(npts / (ms_f64 / 1e3), 1), "unit": "points/s",
+                     "ms_per_step": round(ms_f64, 3), "roofline": roof64},
+        "mixed_over_fp64": round(ms_f64 / ms_mixed, 3),
+        "error": {"plane_flux_rel": flux_err, "plane_flux_fp64": f64.tolist(),
+                  "plane_flux_mixed": fmx.tolist(), "tensor_rel_max": float(e.max()),
+                  "tensor_rel_p99": float(np.quantile(e, 0.99)),
+                  "tensor_sample_points": int(len(e))},
+        "kernel_ms": {k: round(v[0], 3) for k, v in ktimes.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "wall_s": {"mixed": round(wall_mixed, 3), "fp64": round(wall_f64, 3)},
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,755 @@
+// C-ABI entry points: the reference's drop-in gimic_* functions on an implicit
+// global context, and the batched handle API (include/gimic_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gimic_hip.h"
+#include "device_common.h"
+#include "host_basis.h"
+#include "quadrature.h"
+
+using namespace gimic;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw Error(e == hipErrorOutOfMemory ? GIMIC_ENOMEM : GIMIC_EDEVICE,
+                    std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        g_error.clear();
+        return GIMIC_OK;
+    } catch (const Error &e) {
+        g_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_error = "host allocation failed";
+        return GIMIC_ENOMEM;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return GIMIC_EIO;
+    }
+}
+
+// Owned device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n, hipStream_t s, bool zero = true) {
+        if (n <= bytes && p) {
+            if (zero) hip_check(hipMemsetAsync(p, 0, n, s), "hipMemsetAsync");
+            return;
+        }
+        release();
+        hip_check(hipMalloc(&p, std::max<size_t>(n, 256)), "hipMalloc");
+        bytes = n;
+        if (zero) hip_check(hipMemsetAsync(p, 0, n, s), "hipMemsetAsync");
+    }
+    template <class T>
+    T *as() const { return (T *)p; }
+};
+
+size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+uint16_t host_bf16(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+float host_bf16_f(uint16_t h) {
+    uint32_t u = ((uint32_t)h) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+}  // namespace
+
+struct gimic_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Basis basis;
+    double screen_thrs = 1e-6;
+    int giao = 1, diamag = 1, paramag = 1;
+    int nset = 0;
+    std::vector<double> dens;  // nset*4*n*n column-major, as given
+    int lda = 0, ncol_pad = 0, ntiles = 0;
+    DevBuf shells, exps, ncc, coords, rf64, rf32;
+    // right operands per (spin list, precision)
+    struct Mats {
+        DevBuf b64, bhi, blo;
+        int nmatset = 0;
+    };
+    std::map<std::pair<std::vector<int>, int>, std::unique_ptr<Mats>> mats;
+    // workspace
+    DevBuf phi64, grad64, ahi, alo, phi32, grad32, part, pts_tmp, jt_tmp, contrib, planes,
+        pparams, offs, outs;
+    // optional per-kernel timing with HIP events on the launch stream:
+    // [0] basis evaluation, [1] contraction, [2] finalize
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
+    double ms[3] = {0.0, 0.0, 0.0};
+    long long launches[3] = {0, 0, 0};
+
+    ~gimic_ctx() {
+        for (auto &v : ev)
+            for (auto &e : v) {
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+};
+
+namespace {
+
+void setup_device(gimic_ctx *c) {
+    hip_check(hipGetDevice(&c->device), "hipGetDevice");
+    hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
+    Basis &b = c->basis;
+    finalize_basis(b, c->screen_thrs);
+    c->lda = (int)round_up((size_t)b.nbf, kKPad);
+    c->ncol_pad = (int)round_up((size_t)b.nbf, kNT);
+    c->ntiles = c->ncol_pad / kNT;
+    std::vector<DevShell> sh;
+    for (auto &s : b.shells)
+        sh.push_back(DevShell{s.atom, s.l, s.nprim, s.poff, s.fout,
+                              b.spherical ? 2 * s.l + 1 : ncart(s.l), s.radius});
+    hipStream_t st = c->stream;
+    auto up = [&](DevBuf &d, const void *src, size_t n) {
+        d.alloc(n, st, false);
+        hip_check(hipMemcpyAsync(d.p, src, n, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    };
+    up(c->shells, sh.data(), sh.size() * sizeof(DevShell));
+    up(c->exps, b.exps.data(), b.exps.size() * sizeof(double));
+    up(c->ncc, b.ncc.data(), b.ncc.size() * sizeof(double));
+    up(c->coords, b.coords.data(), b.coords.size() * sizeof(double));
+    std::vector<double> rf(3 * (size_t)c->ncol_pad, 0.0);
+    std::vector<float> rf32(3 * (size_t)c->ncol_pad, 0.f);
+    for (auto &s : b.shells) {
+        const int nout = b.spherical ? 2 * s.l + 1 : ncart(s.l);
+        for (int f = 0; f < nout; f++)
+            for (int d = 0; d < 3; d++) {
+                rf[(size_t)d * c->ncol_pad + s.fout + f] = b.coords[3 * s.atom + d];
+                rf32[(size_t)d * c->ncol_pad + s.fout + f] = (float)b.coords[3 * s.atom + d];
+            }
+    }
+    up(c->rf64, rf.data(), rf.size() * sizeof(double));
+    up(c->rf32, rf32.data(), rf32.size() * sizeof(float));
+    std::vector<double> sph;
+    for (int l = 2; l <= kDevMaxL; l++) {
+        auto T = sph_transform(l);
+        sph.insert(sph.end(), T.begin(), T.end());
+    }
+    hip_check(upload_basis_tables(sph.data(), (int)sph.size(), st), "upload_basis_tables");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+// FP64 density family of one spin case: 4 n*n column-major blocks.
+std::vector<double> spin_family(const gimic_ctx *c, int spin) {
+    const size_t n = (size_t)c->basis.nbf, blk = 4 * n * n;
+    if (c->nset == 0) throw Error(GIMIC_ESTATE, "no density set");
+    if (c->nset == 1) {
+        if (spin == GIMIC_SPIN_BETA)
+            throw Error(GIMIC_ESTATE, "beta current requested, but not open-shell system!");
+        if (spin == GIMIC_SPIN_SPINDENS)
+            throw Error(GIMIC_ESTATE, "spindens requested, but not open-shell system!");
+        return std::vector<double>(c->dens.begin(), c->dens.begin() + blk);
+    }
+    std::vector<double> out(blk);
+    const double *a = c->dens.data(), *bb = c->dens.data() + blk;
+    for (size_t i = 0; i < blk; i++) {
+        switch (spin) {
+            case GIMIC_SPIN_ALPHA: out[i] = a[i]; break;
+            case GIMIC_SPIN_BETA: out[i] = bb[i]; break;
+            case GIMIC_SPIN_SPINDENS: out[i] = a[i] - bb[i]; break;
+            default: out[i] = a[i] + bb[i]; break;
+        }
+    }
+    return out;
+}
+
+gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec) {
+    auto key = std::make_pair(spins, prec);
+    auto it = c->mats.find(key);
+    if (it != c->mats.end()) return *it->second;
+    const size_t n = (size_t)c->basis.nbf, ld = (size_t)c->lda, nc = (size_t)c->ncol_pad;
+    const size_t matsz = nc * ld, setsz = kNMat * matsz;
+    auto m = std::make_unique<gimic_ctx::Mats>();
+    m->nmatset = (int)spins.size();
+    std::vector<double> B(setsz * spins.size(), 0.0);
+    std::vector<int> atom_of(n);
+    for (auto &s : c->basis.shells) {
+        const int nout = c->basis.spherical ? 2 * s.l + 1 : ncart(s.l);
+        for (int f = 0; f < nout; f++) atom_of[s.fout + f] = s.atom;
+    }
+    for (size_t si = 0; si < spins.size(); si++) {
+        std::vector<double> fam = spin_family(c, spins[si]);
+        double *Bs = &B[si * setsz];
+        // matrix j column nu (K = mu contiguous): D(mu,nu) = fam[blk + mu + n*nu]
+        for (size_t nu = 0; nu < n; nu++)
+            for (size_t mu = 0; mu < n; mu++) {
+                const double D = fam[mu + n * nu];
+                Bs[0 * matsz + nu * ld + mu] = D;
+                for (int b = 1; b < 4; b++)
+                    Bs[b * matsz + nu * ld + mu] = fam[(size_t)b * n * n + mu + n * nu];
+                for (int d = 0; d < 3; d++)
+                    Bs[(4 + d) * matsz + nu * ld + mu] =
+                        c->basis.coords[3 * atom_of[mu] + d] * D;
+            }
+    }
+    hipStream_t st = c->stream;
+    if (prec == GIMIC_PREC_FP64) {
+        m->b64.alloc(B.size() * sizeof(double), st, false);
+        hip_check(hipMemcpyAsync(m->b64.p, B.data(), B.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, st),
+                  "hipMemcpyAsync");
+    } else {
+        std::vector<uint16_t> hi(B.size()), lo(B.size());
+        for (size_t i = 0; i < B.size(); i++) {
+            const float x = (float)B[i];
+            hi[i] = host_bf16(x);
+            lo[i] = host_bf16(x - host_bf16_f(hi[i]));
+        }
+        m->bhi.alloc(hi.size() * 2, st, false);
+        m->blo.alloc(lo.size() * 2, st, false);
+        hip_check(hipMemcpyAsync(m->bhi.p, hi.data(), hi.size() * 2, hipMemcpyHostToDevice, st),
+                  "hipMemcpyAsync");
+        hip_check(hipMemcpyAsync(m->blo.p, lo.data(), lo.size() * 2, hipMemcpyHostToDevice, st),
+                  "hipMemcpyAsync");
+    }
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    auto &ref = *m;
+    c->mats[key] = std::move(m);
+    return ref;
+}
+
+// Run `launch` between two events on stream st when timing is on.
+template <class F>
+void timed(gimic_ctx *c, int which, hipStream_t st, F &&launch) {
+    if (!c->timing) {
+        launch();
+        return;
+    }
+    hipEvent_t a, b;
+    hip_check(hipEventCreate(&a), "hipEventCreate");
+    hip_check(hipEventCreate(&b), "hipEventCreate");
+    c->ev[which].push_back({a, b});
+    hip_check(hipEventRecord(a, st), "hipEventRecord");
+    launch();
+    hip_check(hipEventRecord(b, st), "hipEventRecord");
+}
+
+struct FluxDev {
+    const double *w = nullptr;
+    const int *plane = nullptr;
+    const double *pnorm = nullptr, *pbfield = nullptr, *pcenter = nullptr, *pradius = nullptr;
+    double *contrib = nullptr;
+    double *contrib_mod = nullptr;
+};
+
+// Evaluate J for npts device points into d_jt (nspin blocks of 9*npts).
+void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
+              const std::vector<int> &spins, int prec, hipStream_t st, const FluxDev *flux) {
+    if (npts <= 0) return;
+    if (prec != GIMIC_PREC_FP64 && prec != GIMIC_PREC_BF16X3)
+        throw Error(GIMIC_EINVAL, "unknown precision");
+    for (int s : spins)
+        if (s < 0 || s > 3) throw Error(GIMIC_EINVAL, "invalid spin case");
+    auto &M = get_mats(c, spins, prec);
+    const size_t ld = (size_t)c->lda;
+    // chunk of points per pass: basis values + gradients of one chunk stay
+    // within ~4 GiB (FP64: 32 bytes per point and function)
+    const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 20);
+    size_t cap = (size_t)4 << 30;
+    size_t chunk = std::max<size_t>(kWgRows, (cap / per_row) / kWgRows * kWgRows);
+    chunk = std::min(chunk, round_up((size_t)npts, kWgRows));
+    const size_t rows_cap = chunk;
+    const long long gplane = (long long)(rows_cap * ld);
+    if (prec == GIMIC_PREC_FP64) {
+        c->phi64.alloc(rows_cap * ld * 8, st, c->phi64.bytes < rows_cap * ld * 8);
+        c->grad64.alloc(3 * rows_cap * ld * 8, st, c->grad64.bytes < 3 * rows_cap * ld * 8);
+    } else {
+        c->ahi.alloc(rows_cap * ld * 2, st, c->ahi.bytes < rows_cap * ld * 2);
+        c->alo.alloc(rows_cap * ld * 2, st, c->alo.bytes < rows_cap * ld * 2);
+        c->phi32.alloc(rows_cap * ld * 4, st, c->phi32.bytes < rows_cap * ld * 4);
+        c->grad32.alloc(3 * rows_cap * ld * 4, st, c->grad32.bytes < 3 * rows_cap * ld * 4);
+    }
+    const int nmatset = M.nmatset;
+    for (size_t start = 0; start < (size_t)npts; start += chunk) {
+        const int n_c = (int)std::min(chunk, (size_t)npts - start);
+        const int rows = (int)round_up((size_t)n_c, kWgRows);
+        const int wg_rows = rows / kWgRows;
+        int nsplit = (int)std::min<long long>(
+            c->ntiles, std::max<long long>(1, (2048 + (long long)wg_rows * nmatset - 1) /
+                                                  ((long long)wg_rows * nmatset)));
+        const int tps = (c->ntiles + nsplit - 1) / nsplit;
+        nsplit = (c->ntiles + tps - 1) / tps;
+        c->part.alloc((size_t)nmatset * nsplit * kNQ * rows * 8, st, false);
+
+        EvalArgs ea{};
+        ea.pts = d_r + 3 * start;
+        ea.npts = n_c;
+        ea.rows = rows;
+        ea.gplane = gplane;
+        ea.shells = c->shells.as<DevShell>();
+        ea.nshell = (int)c->basis.shells.size();
+        ea.exps = c->exps.as<double>();
+        ea.ncc = c->ncc.as<double>();
+        ea.coords = c->coords.as<double>();
+        ea.turbomole = c->basis.turbomole;
+        ea.spherical = c->basis.spherical;
+        ea.lda = c->lda;
+        ea.phi64 = c->phi64.as<double>();
+        ea.grad64 = c->grad64.as<double>();
+        ea.ahi = c->ahi.as<uint16_t>();
+        ea.alo = c->alo.as<uint16_t>();
+        ea.phi32 = c->phi32.as<float>();
+        ea.grad32 = c->grad32.as<float>();
+        timed(c, 0, st, [&] {
+            hip_check(launch_basis_eval(ea, prec == GIMIC_PREC_FP64 ? 0 : 1, st), "basis_eval");
+        });
+
+        ContractArgs ca{};
+        ca.rows = rows;
+        ca.lda = c->lda;
+        ca.ncol_pad = c->ncol_pad;
+        ca.gplane = gplane;
+        ca.ntiles = c->ntiles;
+        ca.nsplit = nsplit;
+        ca.tiles_per_split = tps;
+        ca.nmatset = nmatset;
+        ca.a64 = c->phi64.as<double>();
+        ca.g64 = c->grad64.as<double>();
+        ca.b64 = M.b64.as<double>();
+        ca.ahi = c->ahi.as<uint16_t>();
+        ca.alo = c->alo.as<uint16_t>();
+        ca.bhi = M.bhi.as<uint16_t>();
+        ca.blo = M.blo.as<uint16_t>();
+        ca.phi32 = c->phi32.as<float>();
+        ca.g32 = c->grad32.as<float>();
+        ca.rf64 = c->rf64.as<double>();
+        ca.rf32 = c->rf32.as<float>();
+        ca.part = c->part.as<double>();
+        timed(c, 1, st, [&] {
+            if (prec == GIMIC_PREC_FP64)
+                hip_check(launch_contract_f64(ca, st), "contract_f64");
+            else
+                hip_check(launch_contract_bf16x3(ca, st), "contract_bf16x3");
+        });
+
+        FinalizeArgs fa{};
+        fa.part = c->part.as<double>();
+        fa.nsplit = nsplit;
+        fa.rows = rows;
+        fa.npts = n_c;
+        fa.nmatset = nmatset;
+        fa.pts = d_r + 3 * start;
+        fa.giao = c->giao;
+        fa.diamag = c->diamag;
+        fa.paramag = c->paramag;
+        fa.jt = d_jt + 9 * start;
+        fa.jt_stride = 9 * (long long)npts;
+        if (flux) {
+            fa.w = flux->w + start;
+            fa.plane = flux->plane ? flux->plane + start : nullptr;
+            fa.pnorm = flux->pnorm;
+            fa.pbfield = flux->pbfield;
+            fa.pcenter = flux->pcenter;
+            fa.pradius = flux->pradius;
+            fa.contrib = flux->contrib + start;
+            fa.contrib_mod = flux->contrib_mod ? flux->contrib_mod + start : nullptr;
+        }
+        timed(c, 2, st, [&] { hip_check(launch_finalize(fa, st), "finalize"); });
+    }
+}
+
+// ---- drop-in global state (gimic_interface.f90:21-24) ----
+gimic_ctx *g_ctx = nullptr;
+double g_magnet[3] = {0.0, 0.0, 0.0};
+int g_spin = GIMIC_SPIN_TOTAL;
+int g_uhf_next = 0;
+double g_thrs_next = 1e-6;  // SCREEN_THRS, globals.f90:58
+
+void report(const char *where) {
+    if (!g_error.empty()) std::fprintf(stderr, "gimic: %s: %s\n", where, g_error.c_str());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gimic_last_error(void) { return g_error.c_str(); }
+
+int gimic_ctx_create_files(gimic_ctx **out, const char *molfile, const char *densfile, int uhf,
+                           double screen_thrs) {
+    return guarded([&] {
+        if (!out || !molfile) throw Error(GIMIC_EINVAL, "null argument");
+        auto c = std::make_unique<gimic_ctx>();
+        c->basis = read_mol(molfile);
+        c->screen_thrs = screen_thrs;
+        if (densfile) {
+            c->nset = uhf ? 2 : 1;
+            c->dens = read_xdens(densfile, c->basis, c->nset, uhf != 0);
+        }
+        setup_device(c.get());
+        *out = c.release();
+    });
+}
+
+int gimic_ctx_create(gimic_ctx **out, int natoms, const double *coords, int nshell,
+                     const int *shell_atom, const int *shell_l, const int *shell_nprim,
+                     const int *shell_poff, const double *exps, const double *coefs,
+                     int turbomole, int spherical, double screen_thrs) {
+    return guarded([&] {
+        if (!out || !coords || !shell_atom || !shell_l || !shell_nprim || !shell_poff || !exps ||
+            !coefs)
+            throw Error(GIMIC_EINVAL, "null argument");
+        auto c = std::make_unique<gimic_ctx>();
+        try {
+            c->basis = make_basis(natoms, coords, nshell, shell_atom, shell_l, shell_nprim,
+                                  shell_poff, exps, coefs, turbomole != 0, spherical != 0);
+        } catch (const std::runtime_error &e) {
+            throw Error(GIMIC_EINVAL, e.what());
+        }
+        c->screen_thrs = screen_thrs;
+        setup_device(c.get());
+        *out = c.release();
+    });
+}
+
+int gimic_ctx_set_density(gimic_ctx *c, int nset, const double *dens) {
+    return guarded([&] {
+        if (!c || !dens || (nset != 1 && nset != 2)) throw Error(GIMIC_EINVAL, "bad argument");
+        const size_t n = (size_t)c->basis.nbf;
+        c->nset = nset;
+        c->dens.assign(dens, dens + (size_t)nset * 4 * n * n);
+        c->mats.clear();
+    });
+}
+
+int gimic_ctx_set_flags(gimic_ctx *c, int giao, int diamag, int paramag) {
+    return guarded([&] {
+        if (!c) throw Error(GIMIC_EINVAL, "null context");
+        c->giao = giao != 0;
+        c->diamag = diamag != 0;
+        c->paramag = paramag != 0;
+    });
+}
+
+int gimic_ctx_info(const gimic_ctx *c, int *nbf, int *nbf_cart, int *natoms, int *nshell,
+                   int *nset) {
+    return guarded([&] {
+        if (!c) throw Error(GIMIC_EINVAL, "null context");
+        if (nbf) *nbf = c->basis.nbf;
+        if (nbf_cart) *nbf_cart = c->basis.nbf_cart;
+        if (natoms) *natoms = c->basis.natoms();
+        if (nshell) *nshell = (int)c->basis.shells.size();
+        if (nset) *nset = c->nset;
+    });
+}
+
+int gimic_ctx_atoms(const gimic_ctx *c, double *coords, double *charges) {
+    return guarded([&] {
+        if (!c) throw Error(GIMIC_EINVAL, "null context");
+        if (coords) std::memcpy(coords, c->basis.coords.data(), c->basis.coords.size() * 8);
+        if (charges) std::memcpy(charges, c->basis.charges.data(), c->basis.charges.size() * 8);
+    });
+}
+
+int gimic_ctx_set_timing(gimic_ctx *c, int on) {
+    return guarded([&] {
+        if (!c) throw Error(GIMIC_EINVAL, "null context");
+        c->timing = on != 0;
+        for (int k = 0; k < 3; k++) {
+            for (auto &e : c->ev[k]) {
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
+            c->ev[k].clear();
+            c->ms[k] = 0.0;
+            c->launches[k] = 0;
+        }
+    });
+}
+
+int gimic_ctx_get_timing(gimic_ctx *c, double *ms, long long *launches) {
+    return guarded([&] {
+        if (!c || !ms || !launches) throw Error(GIMIC_EINVAL, "null argument");
+        for (int k = 0; k < 3; k++) {
+            for (auto &e : c->ev[k]) {
+                hip_check(hipEventSynchronize(e.second), "hipEventSynchronize");
+                float t = 0.f;
+                hip_check(hipEventElapsedTime(&t, e.first, e.second), "hipEventElapsedTime");
+                c->ms[k] += t;
+                c->launches[k] += 1;
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
+            c->ev[k].clear();
+            ms[k] = c->ms[k];
+            launches[k] = c->launches[k];
+        }
+    });
+}
+
+int gimic_ctx_destroy(gimic_ctx *c) {
+    return guarded([&] { delete c; });
+}
+
+int gimic_jtensor_multi_device(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
+                               int nspin, const int *spins, int precision, void *stream) {
+    return guarded([&] {
+        if (!c || npts < 0 || (npts > 0 && (!d_r || !d_jt)) || nspin < 1 || !spins)
+            throw Error(GIMIC_EINVAL, "bad argument");
+        hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+        evaluate(c, npts, d_r, d_jt, std::vector<int>(spins, spins + nspin), precision, st,
+                 nullptr);
+    });
+}
+
+int gimic_jtensor_batch_device(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
+                               int spin, int precision, void *stream) {
+    return gimic_jtensor_multi_device(c, npts, d_r, d_jt, 1, &spin, precision, stream);
+}
+
+int gimic_jtensor_batch(gimic_ctx *c, int64_t npts, const double *r, double *jt, int spin,
+                        int precision) {
+    return guarded([&] {
+        if (!c || npts < 0 || (npts > 0 && (!r || !jt))) throw Error(GIMIC_EINVAL, "bad argument");
+        if (npts == 0) return;
+        hipStream_t st = c->stream;
+        c->pts_tmp.alloc((size_t)npts * 3 * 8, st, false);
+        c->jt_tmp.alloc((size_t)npts * 9 * 8, st, false);
+        hip_check(hipMemcpyAsync(c->pts_tmp.p, r, (size_t)npts * 24, hipMemcpyHostToDevice, st),
+                  "hipMemcpyAsync");
+        evaluate(c, npts, c->pts_tmp.as<double>(), c->jt_tmp.as<double>(), {spin}, precision, st,
+                 nullptr);
+        hip_check(hipMemcpyAsync(jt, c->jt_tmp.p, (size_t)npts * 72, hipMemcpyDeviceToHost, st),
+                  "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    });
+}
+
+int gimic_flux_planes(gimic_ctx *c, int nplanes, const int64_t *offs, const double *r,
+                      const double *w, const double *normals, const double *bfields,
+                      const double *centers, const double *radii, int spin, int precision,
+                      double *out, double *out_mod, double *jt) {
+    return guarded([&] {
+        if (!c || nplanes < 1 || !offs || !r || !w || !normals || !bfields || !centers ||
+            !radii || !out)
+            throw Error(GIMIC_EINVAL, "null argument");
+        if (offs[0] != 0) throw Error(GIMIC_EINVAL, "plane_offs[0] must be 0");
+        for (int k = 0; k < nplanes; k++)
+            if (offs[k + 1] < offs[k]) throw Error(GIMIC_EINVAL, "plane_offs must be non-decreasing");
+        const int64_t npts = offs[nplanes];
+        hipStream_t st = c->stream;
+        std::vector<int> plane(npts);
+        for (int k = 0; k < nplanes; k++)
+            for (int64_t i = offs[k]; i < offs[k + 1]; i++) plane[i] = k;
+        std::vector<double> params(10 * (size_t)nplanes);
+        for (int k = 0; k < nplanes; k++) {
+            for (int d = 0; d < 3; d++) {
+                params[3 * k + d] = normals[3 * k + d];
+                params[3 * nplanes + 3 * k + d] = bfields[3 * k + d];
+                params[6 * nplanes + 3 * k + d] = centers[3 * k + d];
+            }
+            params[9 * nplanes + k] = radii[k];
+        }
+        const size_t np = (size_t)std::max<int64_t>(npts, 1);
+        c->pts_tmp.alloc(np * 24 + np * 8, st, false);  // points then weights
+        c->jt_tmp.alloc(np * 72, st, false);
+        c->contrib.alloc(np * 16, st, true);  // current, then signed modulus
+        c->planes.alloc(np * 4, st, false);
+        c->pparams.alloc(params.size() * 8, st, false);
+        c->offs.alloc((size_t)(nplanes + 1) * 8, st, false);
+        c->outs.alloc((size_t)nplanes * 48, st, false);
+        double *d_r = c->pts_tmp.as<double>(), *d_w = d_r + 3 * np;
+        hip_check(hipMemcpyAsync(d_r, r, (size_t)npts * 24, hipMemcpyHostToDevice, st), "copy r");
+        hip_check(hipMemcpyAsync(d_w, w, (size_t)npts * 8, hipMemcpyHostToDevice, st), "copy w");
+        hip_check(hipMemcpyAsync(c->planes.p, plane.data(), (size_t)npts * 4,
+                                 hipMemcpyHostToDevice, st), "copy planes");
+        hip_check(hipMemcpyAsync(c->pparams.p, params.data(), params.size() * 8,
+                                 hipMemcpyHostToDevice, st), "copy params");
+        hip_check(hipMemcpyAsync(c->offs.p, offs, (size_t)(nplanes + 1) * 8,
+                                 hipMemcpyHostToDevice, st), "copy offs");
+        FluxDev fd;
+        fd.w = d_w;
+        fd.plane = c->planes.as<int>();
+        const double *pp = c->pparams.as<double>();
+        fd.pnorm = pp;
+        fd.pbfield = pp + 3 * nplanes;
+        fd.pcenter = pp + 6 * nplanes;
+        fd.pradius = pp + 9 * nplanes;
+        fd.contrib = c->contrib.as<double>();
+        fd.contrib_mod = out_mod ? c->contrib.as<double>() + np : nullptr;
+        evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd);
+        hip_check(launch_plane_reduce(c->contrib.as<double>(), (const long long *)c->offs.p,
+                                      nplanes, c->outs.as<double>(), st),
+                  "plane_reduce");
+        hip_check(hipMemcpyAsync(out, c->outs.p, (size_t)nplanes * 24, hipMemcpyDeviceToHost, st),
+                  "copy out");
+        if (out_mod) {
+            hip_check(launch_plane_reduce(c->contrib.as<double>() + np,
+                                          (const long long *)c->offs.p, nplanes,
+                                          c->outs.as<double>() + 3 * nplanes, st),
+                      "plane_reduce");
+            hip_check(hipMemcpyAsync(out_mod, c->outs.as<double>() + 3 * nplanes,
+                                     (size_t)nplanes * 24, hipMemcpyDeviceToHost, st),
+                      "copy out_mod");
+        }
+        if (jt)
+            hip_check(hipMemcpyAsync(jt, c->jt_tmp.p, (size_t)npts * 72, hipMemcpyDeviceToHost,
+                                     st), "copy jt");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    });
+}
+
+int gimic_sph_transform(int l, double *T) {
+    return guarded([&] {
+        if (!T || l < 0 || l > kMaxL) throw Error(GIMIC_EINVAL, "bad argument");
+        if (l <= 1) {  // s and p functions are used as they are (x, y, z)
+            const int n = ncart(l);
+            for (int i = 0; i < n * n; i++) T[i] = (i % (n + 1) == 0) ? 1.0 : 0.0;
+            return;
+        }
+        auto v = sph_transform(l);
+        std::memcpy(T, v.data(), v.size() * sizeof(double));
+    });
+}
+
+int gimic_selftest_mfma_layout(double *f64, float *bf16) {
+    return guarded([&] {
+        if (!f64 || !bf16) throw Error(GIMIC_EINVAL, "null argument");
+        hip_check(run_mfma_layout_probe(f64, bf16), "mfma layout probe");
+    });
+}
+
+int gimic_quadrature(double a, double b, int order, int npts, int kind, double *pts,
+                     double *wgts) {
+    return guarded([&] {
+        if (!pts || !wgts || npts < 1 || order < 1 || (kind != 0 && kind != 1))
+            throw Error(GIMIC_EINVAL, "bad argument");
+        if (npts > 1 && npts % order != 0)
+            throw Error(GIMIC_EINVAL, "npts is not dividable by the quadrature order");
+        setup_gauss_data(a, b, order, npts, kind == 1, pts, wgts);
+    });
+}
+
+// ---------------- drop-in (gimic_interface.f90) ----------------
+
+void gimic_init(const char *molfile, const char *densfile) {
+    if (g_ctx) {
+        delete g_ctx;
+        g_ctx = nullptr;
+    }
+    g_spin = GIMIC_SPIN_TOTAL;
+    g_magnet[0] = g_magnet[1] = g_magnet[2] = 0.0;
+    if (gimic_ctx_create_files(&g_ctx, molfile, densfile, g_uhf_next, g_thrs_next) != GIMIC_OK) {
+        g_ctx = nullptr;
+        report("gimic_init");
+    }
+}
+
+void gimic_finalize(void) {
+    delete g_ctx;
+    g_ctx = nullptr;
+}
+
+void gimic_set_uhf(int *uhf) { g_uhf_next = (uhf && *uhf != 0) ? 1 : 0; }
+
+void gimic_set_magnet(const double *b) {
+    if (b) std::memcpy(g_magnet, b, sizeof g_magnet);
+}
+
+void gimic_set_spin(const char *s) {
+    const std::string v = s ? s : "";
+    if (v == "alpha") g_spin = GIMIC_SPIN_ALPHA;
+    else if (v == "beta") g_spin = GIMIC_SPIN_BETA;
+    else if (v == "total") g_spin = GIMIC_SPIN_TOTAL;
+    else if (v == "spindens") g_spin = GIMIC_SPIN_SPINDENS;
+    else {
+        g_error = "Invalid spin case.";
+        report("gimic_set_spin");
+    }
+}
+
+void gimic_set_screening(const double *thrs) {
+    if (thrs) g_thrs_next = *thrs;
+}
+
+void gimic_calc_jtensor(const double *r, double *jt) {
+    if (!jt) return;
+    if (!g_ctx) {
+        g_error = "gimic_init has not been called";
+        report("gimic_calc_jtensor");
+        for (int i = 0; i < 9; i++) jt[i] = 0.0;
+        return;
+    }
+    if (gimic_jtensor_batch(g_ctx, 1, r, jt, g_spin, GIMIC_PREC_FP64) != GIMIC_OK) {
+        report("gimic_calc_jtensor");
+        for (int i = 0; i < 9; i++) jt[i] = 0.0;
+    }
+}
+
+void gimic_calc_jvector(const double *r, double *jv) {
+    double jt[9];
+    gimic_calc_jtensor(r, jt);
+    if (!jv) return;
+    // jv = matmul(reshape(j,(3,3)), B) (jtensor.F90:125-135)
+    for (int m = 0; m < 3; m++)
+        jv[m] = jt[m] * g_magnet[0] + jt[m + 3] * g_magnet[1] + jt[m + 6] * g_magnet[2];
+}
+
+void gimic_calc_modj(const double *r, double *mj) {
+    double jv[3];
+    gimic_calc_jvector(r, jv);
+    if (mj) *mj = std::sqrt(jv[0] * jv[0] + jv[1] * jv[1] + jv[2] * jv[2]);
+}
+
+void gimic_get_gauss_points(double *a, double *b, int *npts, int *order, double *pts,
+                            double *wgts) {
+    if (!a || !b || !npts || !order) return;
+    if (gimic_quadrature(*a, *b, *order, *npts, 0, pts, wgts) != GIMIC_OK)
+        report("gimic_get_gauss_points");
+}
+
+void mkgausspoints(double *a, double *b, int *npts, int *order, double *pts, double *wgts) {
+    gimic_get_gauss_points(a, b, npts, order, pts, wgts);
+}
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+// Basis-function evaluation per point tile (gfx950).
+//
+// One thread per (point, contraction), contraction index fastest, so a
+// wavefront writes neighbouring function columns of one point row.  Each
+// thread evaluates the contracted radial parts g = sum c e^{-a s} and
+// h = sum a c e^{-a s} once (the reference recomputes the exponentials for
+// every derivative axis, caos.f90:39-64) and forms, per cartesian component
+// x^a y^b z^c (gtodefs.f90:86-123):
+//   phi    = x^a y^b z^c g                                (caos.f90:17-37)
+//   d_m phi = k_m r_m^(k_m-1)(...) g - 2 r_m x^a y^b z^c h (caos.f90:39-64)
+// with contractions outside their screening radius set to zero
+// (basis.f90:118-136).  Spherical contractions are T_l times the cartesian
+// ones.  Outputs are written as the contraction's left operand (FP64, or a
+// BF16 hi/lo split) plus phi and the gradient for the epilogue.
+#include "device_common.h"
+
+namespace gimic {
+
+__constant__ double c_sph[466];  // T_l, l = 2..5, (2l+1) x ncart(l) row-major
+
+struct Comp {
+    signed char e[21][3];
+};
+
+__host__ __device__ constexpr Comp lex_table(int L) {
+    Comp t{};
+    int k = 0;
+    for (int a = L; a >= 0; a--)
+        for (int b = L - a; b >= 0; b--) {
+            t.e[k][0] = (signed char)a;
+            t.e[k][1] = (signed char)b;
+            t.e[k][2] = (signed char)(L - a - b);
+            k++;
+        }
+    return t;
+}
+
+// TURBOMOLE component order, gtodefs.f90:108-123
+__host__ __device__ constexpr Comp tm_table(int L) {
+    Comp t{};
+    if (L == 2) {
+        const signed char d[6][3] = {{2,0,0},{0,2,0},{0,0,2},{1,1,0},{1,0,1},{0,1,1}};
+        for (int i = 0; i < 6; i++) for (int k = 0; k < 3; k++) t.e[i][k] = d[i][k];
+    } else if (L == 3) {
+        const signed char f[10][3] = {{3,0,0},{0,3,0},{0,0,3},{2,1,0},{2,0,1},
+                                      {1,2,0},{0,2,1},{1,0,2},{0,1,2},{1,1,1}};
+        for (int i = 0; i < 10; i++) for (int k = 0; k < 3; k++) t.e[i][k] = f[i][k];
+    } else if (L == 4) {
+        const signed char g[15][3] = {{4,0,0},{0,4,0},{0,0,4},{3,1,0},{3,0,1},{1,3,0},
+                                      {0,3,1},{1,0,3},{0,1,3},{2,2,0},{2,0,2},{0,2,2},
+                                      {2,1,1},{1,2,1},{1,1,2}};
+        for (int i = 0; i < 15; i++) for (int k = 0; k < 3; k++) t.e[i][k] = g[i][k];
+    } else if (L == 5) {
+        const signed char h[21][3] = {{5,0,0},{0,5,0},{0,0,5},{4,1,0},{4,0,1},{1,4,0},
+                                      {0,4,1},{1,0,4},{0,1,4},{3,2,0},{3,0,2},{2,3,0},
+                                      {0,3,2},{2,0,3},{0,2,3},{3,1,1},{1,3,1},{1,1,3},
+                                      {2,2,1},{2,1,2},{1,2,2}};
+        for (int i = 0; i < 21; i++) for (int k = 0; k < 3; k++) t.e[i][k] = h[i][k];
+    } else {
+        t = lex_table(L);
+    }
+    return t;
+}
+
+__host__ __device__ constexpr int sph_off(int L) {
+    return L == 2 ? 0 : L == 3 ? 30 : L == 4 ? 100 : L == 5 ? 235 : 0;
+}
+
+template <int MODE>
+__device__ __forceinline__ void store(const EvalArgs &a, size_t row, int f, double v,
+                                      double gx, double gy, double gz) {
+    const size_t i = row * (size_t)a.lda + f;
+    const size_t plane = (size_t)a.gplane;
+    if (MODE == 0) {
+        a.phi64[i] = v;
+        a.grad64[i] = gx;
+        a.grad64[plane + i] = gy;
+        a.grad64[2 * plane + i] = gz;
+    } else {
+        const float v32 = (float)v;
+        const uint16_t hi = f32_to_bf16_rn(v32);
+        const uint16_t lo = f32_to_bf16_rn(v32 - bf16_to_f32(hi));
+        a.ahi[i] = hi;
+        a.alo[i] = lo;
+        a.phi32[i] = v32;
+        a.grad32[i] = (float)gx;
+        a.grad32[plane + i] = (float)gy;
+        a.grad32[2 * plane + i] = (float)gz;
+    }
+}
+
+template <int MODE, int L, bool TM, bool SPH>
+__device__ void eval_shell(const EvalArgs &a, size_t row, int fout, const double rr[3], double g,
+                           double h) {
+    constexpr int NC = (L + 1) * (L + 2) / 2;
+    constexpr int NS = 2 * L + 1;
+    constexpr Comp T = TM ? tm_table(L) : lex_table(L);
+    double px[L + 1], py[L + 1], pz[L + 1];
+    px[0] = py[0] = pz[0] = 1.0;
+#pragma unroll
+    for (int i = 1; i <= L; i++) {
+        px[i] = px[i - 1] * rr[0];
+        py[i] = py[i - 1] * rr[1];
+        pz[i] = pz[i - 1] * rr[2];
+    }
+    constexpr bool TRANSFORM = SPH && L >= 2;
+    double acc[TRANSFORM ? NS : 1][4];
+    if (TRANSFORM) {
+#pragma unroll
+        for (int m = 0; m < NS; m++) acc[m][0] = acc[m][1] = acc[m][2] = acc[m][3] = 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int ea = T.e[c][0], eb = T.e[c][1], ec = T.e[c][2];
+        const double mono = px[ea] * py[eb] * pz[ec];
+        const double v = mono * g;
+        const double dx = (ea ? ea * px[ea ? ea - 1 : 0] * py[eb] * pz[ec] * g : 0.0) -
+                          2.0 * rr[0] * mono * h;
+        const double dy = (eb ? eb * px[ea] * py[eb ? eb - 1 : 0] * pz[ec] * g : 0.0) -
+                          2.0 * rr[1] * mono * h;
+        const double dz = (ec ? ec * px[ea] * py[eb] * pz[ec ? ec - 1 : 0] * g : 0.0) -
+                          2.0 * rr[2] * mono * h;
+        if (TRANSFORM) {
+#pragma unroll
+            for (int m = 0; m < NS; m++) {
+                const double t = c_sph[sph_off(L) + m * NC + c];
+                acc[m][0] += t * v;
+                acc[m][1] += t * dx;
+                acc[m][2] += t * dy;
+                acc[m][3] += t * dz;
+            }
+        } else {
+            store<MODE>(a, row, fout + c, v, dx, dy, dz);
+        }
+    }
+    if (TRANSFORM) {
+#pragma unroll
+        for (int m = 0; m < NS; m++)
+            store<MODE>(a, row, fout + m, acc[m][0], acc[m][1], acc[m][2], acc[m][3]);
+    }
+}
+
+template <int MODE, bool TM, bool SPH>
+__device__ void dispatch_l(const EvalArgs &a, size_t row, const DevShell &sh, const double rr[3],
+                           double g, double h) {
+    switch (sh.l) {
+        case 0: eval_shell<MODE, 0, TM, SPH>(a, row, sh.fout, rr, g, h); break;
+        case 1: eval_shell<MODE, 1, TM, SPH>(a, row, sh.fout, rr, g, h); break;
+        case 2: eval_shell<MODE, 2, TM, SPH>(a, row, sh.fout, rr, g, h); break;
+        case 3: eval_shell<MODE, 3, TM, SPH>(a, row, sh.fout, rr, g, h); break;
+        case 4: eval_shell<MODE, 4, TM, SPH>(a, row, sh.fout, rr, g, h); break;
+        default: eval_shell<MODE, 5, TM, SPH>(a, row, sh.fout, rr, g, h); break;
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void basis_eval_kernel(EvalArgs a) {
+    const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long total = (long long)a.rows * a.nshell;
+    if (gid >= total) return;
+    const size_t row = (size_t)(gid / a.nshell);
+    const DevShell sh = a.shells[gid % a.nshell];
+    bool zero = row >= (size_t)a.npts;
+    double rr[3] = {0.0, 0.0, 0.0}, g = 0.0, h = 0.0;
+    if (!zero) {
+        const double *r = a.pts + 3 * row;
+        const double *R = a.coords + 3 * sh.atom;
+        rr[0] = r[0] - R[0];
+        rr[1] = r[1] - R[1];
+        rr[2] = r[2] - R[2];
+        const double s = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
+        zero = sqrt(s) > sh.radius;
+        if (!zero) {
+            for (int p = 0; p < sh.nprim; p++) {
+                const double xp = a.exps[sh.poff + p];
+                const double q = a.ncc[sh.poff + p] * exp(-xp * s);
+                g += q;
+                h += xp * q;
+            }
+        }
+    }
+    if (zero) {
+        for (int f = 0; f < sh.ncomp_out; f++) store<MODE>(a, row, sh.fout + f, 0.0, 0.0, 0.0, 0.0);
+        return;
+    }
+    if (a.turbomole) {
+        if (a.spherical) dispatch_l<MODE, true, true>(a, row, sh, rr, g, h);
+        else dispatch_l<MODE, true, false>(a, row, sh, rr, g, h);
+    } else {
+        if (a.spherical) dispatch_l<MODE, false, true>(a, row, sh, rr, g, h);
+        else dispatch_l<MODE, false, false>(a, row, sh, rr, g, h);
+    }
+}
+
+hipError_t upload_basis_tables(const double *sph, int nsph, hipStream_t s) {
+    if (nsph > 466) return hipErrorInvalidValue;
+    return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sph), sph, sizeof(double) * nsph, 0,
+                                  hipMemcpyHostToDevice, s);
+}
+
+hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
+    const long long total = (long long)a.rows * a.nshell;
+    const unsigned nblk = (unsigned)((total + 255) / 256);
+    if (total <= 0) return hipSuccess;
+    if (mode == 0)
+        hipLaunchKernelGGL(basis_eval_kernel<0>, dim3(nblk), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(basis_eval_kernel<1>, dim3(nblk), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace gimic

@@ -1,0 +1,127 @@
+// FP64 contraction of basis values with the density family on MFMA, with the
+// per-point reductions fused into the epilogue (gfx950).
+//
+// The reference's contract (jtensor.F90:148-237) does, per point, seven
+// length-n GEMVs phi*D, phi*DB_b, db_b*D and 27 dot products.  db_b = S_b phi
+// with S = r x R_A (bfeval.f90:168-293), so every GEMV shares the left
+// operand phi and a tile of P points is one GEMM
+//     Y[P x 7n] = Phi[P x n] * [D | DBx | DBy | DBz | RxD | RyD | RzD]
+// followed by reductions over the output column nu against phi_nu,
+// grad phi_nu and R_nu (the kNQ partial sums listed in device_common.h).
+// The finalize kernel turns the partials into the 3x3 tensor.
+//
+// Tiling: 256 threads = 4 wavefronts; a workgroup owns 128 point rows
+// (32 per wavefront: two 16-row MFMA blocks) and walks a range of 16-column
+// N-tiles; per N-tile each wavefront keeps 2 x 7 v_mfma_f64_16x16x4 blocks.
+// The K index is remapped inside every 16-wide K-step so each lane loads 32
+// contiguous bytes of its A row and of each B column (lane group g holds
+// k = 4g..4g+3; sub-step s feeds k = 4g+s to both operands).
+#include "device_common.h"
+
+namespace gimic {
+
+__global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs a) {
+    __shared__ double qs[kWgRows * kNQ];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
+    const int set = blockIdx.z, split = blockIdx.y;
+    const size_t row0 = (size_t)blockIdx.x * kWgRows;
+    const int lda = a.lda;
+    const size_t rows = (size_t)a.rows;
+    const size_t matsz = (size_t)a.ncol_pad * lda;
+    const size_t gplane = (size_t)a.gplane;
+    for (int i = tid; i < kWgRows * kNQ; i += 256) qs[i] = 0.0;
+    __syncthreads();
+
+    const double *A0 = a.a64 + (row0 + wave * 32 + c) * lda + 4 * g;
+    const double *A1 = A0 + 16 * (size_t)lda;
+    const double *Bset = a.b64 + (size_t)set * kNMat * matsz;
+    const int nt0 = split * a.tiles_per_split;
+    const int nt1 = min(nt0 + a.tiles_per_split, a.ntiles);
+
+    for (int nt = nt0; nt < nt1; nt++) {
+        const int col = nt * kNT + c;
+        const double *Bc = Bset + (size_t)col * lda + 4 * g;
+        v4d acc[2][kNMat];
+#pragma unroll
+        for (int m = 0; m < kNMat; m++) acc[0][m] = acc[1][m] = v4d{0.0, 0.0, 0.0, 0.0};
+
+        v4d a0 = *(const v4d *)(A0), a1 = *(const v4d *)(A1);
+        v4d b[kNMat];
+#pragma unroll
+        for (int m = 0; m < kNMat; m++) b[m] = *(const v4d *)(Bc + m * matsz);
+        for (int k0 = 0; k0 < lda; k0 += 16) {
+            // prefetch the next K-step while this one runs on the matrix cores
+            const int kn = k0 + 16 < lda ? k0 + 16 : k0;
+            const v4d na0 = *(const v4d *)(A0 + kn), na1 = *(const v4d *)(A1 + kn);
+            v4d nb[kNMat];
+#pragma unroll
+            for (int m = 0; m < kNMat; m++) nb[m] = *(const v4d *)(Bc + m * matsz + kn);
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+#pragma unroll
+                for (int m = 0; m < kNMat; m++) {
+                    acc[0][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[s], b[m][s], acc[0][m], 0, 0, 0);
+                    acc[1][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[s], b[m][s], acc[1][m], 0, 0, 0);
+                }
+            }
+            a0 = na0;
+            a1 = na1;
+#pragma unroll
+            for (int m = 0; m < kNMat; m++) b[m] = nb[m];
+        }
+
+        // epilogue: for the f64 16x16x4 builtin, lane (g, c) holds
+        // Y_mat[row g + 4r][col c] of each block (checked by the MFMA
+        // layout self-test)
+        const double R[3] = {a.rf64[col], a.rf64[a.ncol_pad + col], a.rf64[2 * a.ncol_pad + col]};
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rl = wave * 32 + rb * 16 + g + 4 * r;
+                const size_t p = row0 + rl;
+                const double ph = a.a64[p * lda + col];
+                const double G[3] = {a.g64[p * lda + col], a.g64[gplane + p * lda + col],
+                                     a.g64[2 * gplane + p * lda + col]};
+                const double y0 = acc[rb][0][r];
+                double q[kNQ];
+                q[0] = y0 * ph;
+                q[1] = q[0] * R[0];
+                q[2] = q[0] * R[1];
+                q[3] = q[0] * R[2];
+#pragma unroll
+                for (int bb = 0; bb < 3; bb++)
+#pragma unroll
+                    for (int m = 0; m < 3; m++) q[4 + m + 3 * bb] = acc[rb][1 + bb][r] * G[m];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const double t = R[d] * y0 - acc[rb][4 + d][r];
+#pragma unroll
+                    for (int m = 0; m < 3; m++) q[13 + m + 3 * d] = G[m] * t;
+                }
+#pragma unroll
+                for (int off = 1; off < 16; off <<= 1)
+#pragma unroll
+                    for (int i = 0; i < kNQ; i++) q[i] += __shfl_xor(q[i], off);
+#pragma unroll
+                for (int i = 0; i < kNQ; i++)
+                    if ((i & 15) == c) qs[rl * kNQ + i] += q[i];
+            }
+        }
+    }
+    __syncthreads();
+    double *out = a.part + ((size_t)set * a.nsplit + split) * kNQ * rows + row0;
+    for (int i = tid; i < kWgRows * kNQ; i += 256) {
+        const int q = i / kWgRows, rl = i % kWgRows;
+        out[(size_t)q * rows + rl] = qs[rl * kNQ + q];
+    }
+}
+
+hipError_t launch_contract_f64(const ContractArgs &a, hipStream_t s) {
+    dim3 grid(a.rows / kWgRows, a.nsplit, a.nmatset);
+    hipLaunchKernelGGL(contract_f64_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace gimic

@@ -1,0 +1,109 @@
+// Shared device-side definitions for the CDNA4 (gfx950) current-density kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace gimic {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+
+// Number of per-point partial sums produced by the contraction epilogue:
+//   0        rho   = sum_nu Y0 phi
+//   1..3     C_d   = sum_nu Y0 phi R_d
+//   4..12    A_mb  = sum_nu Yb_b G_m            (index 4 + m + 3b)
+//   13..21   Q_md  = sum_nu G_m (R_d Y0 - Z_d)  (index 13 + m + 3d)
+// with Y0 = Phi D, Yb_b = Phi DB_b, Z_d = Phi diag(R_d) D.
+constexpr int kNQ = 22;
+
+// Matrices in the contraction's right operand, per density set.
+constexpr int kNMat = 7;  // D, DBx, DBy, DBz, RxD, RyD, RzD
+
+// Point rows per workgroup tile and columns per N-tile of the contraction.
+constexpr int kWgRows = 128;
+constexpr int kNT = 16;
+// K padding granularity (FP64 K-step 16, BF16 K-step 32)
+constexpr int kKPad = 32;
+
+struct DevShell {
+    int atom;
+    int l;
+    int nprim;
+    int poff;
+    int fout;       // first output function
+    int ncomp_out;  // output functions of the shell
+    double radius;  // screening radius
+};
+
+// Largest l handled by the device kernels (globals.f90:30 MAX_L = 5)
+constexpr int kDevMaxL = 5;
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rn(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+    return __uint_as_float(((uint32_t)h) << 16);
+}
+
+// launchers (defined in the .hip files)
+struct EvalArgs {
+    const double *pts;   // 3*npts of this chunk
+    int npts;            // valid points
+    int rows;            // padded rows (multiple of kWgRows)
+    long long gplane;    // elements between the three gradient planes
+    const DevShell *shells;
+    int nshell;
+    const double *exps, *ncc, *coords;
+    int turbomole, spherical;
+    int lda;             // padded K (multiple of kKPad)
+    // FP64 outputs
+    double *phi64, *grad64;  // [rows][lda], [3][rows][lda]
+    // mixed outputs
+    uint16_t *ahi, *alo;     // [rows][lda] bf16 split of phi
+    float *phi32, *grad32;   // [rows][lda], [3][rows][lda]
+};
+hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s);
+hipError_t upload_basis_tables(const double *sph, int nsph, hipStream_t s);
+
+struct ContractArgs {
+    int rows, lda, ncol_pad;
+    long long gplane;    // elements between the three gradient planes
+    int ntiles, nsplit, tiles_per_split;
+    int nmatset;      // density sets packed side by side (7 matrices each)
+    const double *a64, *g64, *b64;          // FP64 path
+    const uint16_t *ahi, *alo, *bhi, *blo;  // split-BF16 path
+    const float *phi32, *g32;
+    const double *rf64;  // [3][ncol_pad] atom coordinate of each function
+    const float *rf32;
+    double *part;        // [nmatset][nsplit][kNQ][rows]
+};
+hipError_t launch_contract_f64(const ContractArgs &a, hipStream_t s);
+hipError_t launch_contract_bf16x3(const ContractArgs &a, hipStream_t s);
+
+struct FinalizeArgs {
+    const double *part;  // [nmatset][nsplit][kNQ][rows]
+    int nsplit, rows, npts, nmatset;
+    const double *pts;   // chunk points
+    int giao, diamag, paramag;
+    double *jt;          // [nmatset][npts][9] (chunk offset applied)
+    long long jt_stride; // elements between matrix sets in jt
+    // flux (optional): contribution w_p * n_p . (J B_p) per point
+    const double *w;     // weights or null
+    const int *plane;    // plane index per point or null (single plane 0)
+    const double *pnorm, *pbfield, *pcenter, *pradius;  // per plane
+    double *contrib;     // [npts] or null
+    double *contrib_mod; // signed-modulus contribution w_p sgn(n.jv)|jv| or null
+};
+hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s);
+
+// deterministic per-plane sums of contributions: out[3*plane] = net, pos, neg
+hipError_t launch_plane_reduce(const double *contrib, const long long *offs, int nplanes,
+                               double *out, hipStream_t s);
+
+hipError_t run_mfma_layout_probe(double *h_f64, float *h_bf16);
+
+}  // namespace gimic

@@ -1,0 +1,140 @@
+// Tensor assembly from the contraction partials, and plane-flux reductions.
+//
+// J_mb = 1/2 [A_mb + GIAO (eps_bcd r_c Q_md + eps_bmd C_d)] + 1/2 eps_mbc rho r_c
+// reproduces jtensor.F90:163-235 (ZETA = 1/2, globals.f90:47): A is the
+// paramagnetic term (ppd), the GIAO bracket is prsp1 + prsp2, and the last
+// term is the diamagnetic dpd added to the off-diagonal elements.  The
+// paramag switch zeroes the bracket, the diamag switch the last term.
+// All of this runs in FP64 whatever the contraction precision.
+//
+// The flux contribution of a point is w_p n.(J B) (integral.f90:119-143),
+// zero outside the integration radius around the plane centre; the signed
+// modulus contribution is w_p sgn(n.jv) |jv| with jv = J B and a 1e-12 dead
+// zone on the sign (integral.f90:258-283), from the same tensor; per-plane
+// sums (net, positive part, negative part) use a fixed-order tree so results
+// are reproducible run to run.
+#include "device_common.h"
+
+namespace gimic {
+
+__device__ __forceinline__ double eps3(int i, int j, int k) {
+    return (double)((i - j) * (j - k) * (k - i)) * 0.5;
+}
+
+__global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.npts) return;
+    const double r[3] = {a.pts[3 * p], a.pts[3 * p + 1], a.pts[3 * p + 2]};
+    for (int set = 0; set < a.nmatset; set++) {
+        double q[kNQ];
+#pragma unroll
+        for (int i = 0; i < kNQ; i++) q[i] = 0.0;
+        for (int s = 0; s < a.nsplit; s++) {
+            const double *src = a.part + ((size_t)set * a.nsplit + s) * kNQ * (size_t)a.rows + p;
+#pragma unroll
+            for (int i = 0; i < kNQ; i++) q[i] += src[(size_t)i * a.rows];
+        }
+        const double rho = q[0];
+        double J[9];
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                double giao = 0.0;
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const double e = eps3(b, cc, d);
+                        if (e != 0.0) giao += e * r[cc] * q[13 + m + 3 * d];
+                    }
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const double e = eps3(b, m, d);
+                    if (e != 0.0) giao += e * q[1 + d];
+                }
+                double para = q[4 + m + 3 * b];
+                if (a.giao) para += giao;
+                double v = a.paramag ? 0.5 * para : 0.0;
+                if (a.diamag) {
+#pragma unroll
+                    for (int cc = 0; cc < 3; cc++) {
+                        const double e = eps3(m, b, cc);
+                        if (e != 0.0) v += 0.5 * e * rho * r[cc];
+                    }
+                }
+                J[m + 3 * b] = v;
+            }
+        double *o = a.jt + (size_t)set * a.jt_stride + (size_t)p * 9;
+#pragma unroll
+        for (int i = 0; i < 9; i++) o[i] = J[i];
+        if (set == 0 && a.contrib) {
+            const int pl = a.plane ? a.plane[p] : 0;
+            const double *n = a.pnorm + 3 * pl, *B = a.pbfield + 3 * pl, *ctr = a.pcenter + 3 * pl;
+            double dot = 0.0;
+#pragma unroll
+            for (int m = 0; m < 3; m++)
+                dot += n[m] * (J[m] * B[0] + J[m + 3] * B[1] + J[m + 6] * B[2]);
+            const double dx = r[0] - ctr[0], dy = r[1] - ctr[1], dz = r[2] - ctr[2];
+            const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+            const bool out = dist > a.pradius[pl];
+            a.contrib[p] = out ? 0.0 : dot * a.w[p];
+            if (a.contrib_mod) {
+                double jv2 = 0.0;
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    const double v = J[m] * B[0] + J[m + 3] * B[1] + J[m + 6] * B[2];
+                    jv2 += v * v;
+                }
+                const double sgn = fabs(dot) < 1e-12 ? 0.0 : (dot > 0.0 ? 1.0 : -1.0);
+                a.contrib_mod[p] = out ? 0.0 : sgn * sqrt(jv2) * a.w[p];
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void plane_reduce_kernel(const double *contrib,
+                                                           const long long *offs, double *out) {
+    __shared__ double s[3][256];
+    const int pl = blockIdx.x, t = threadIdx.x;
+    const long long beg = offs[pl], end = offs[pl + 1];
+    double net = 0.0, pos = 0.0, neg = 0.0;
+    for (long long i = beg + t; i < end; i += 256) {
+        const double v = contrib[i];
+        net += v;
+        if (v > 0.0) pos += v;
+        else neg += v;
+    }
+    s[0][t] = net;
+    s[1][t] = pos;
+    s[2][t] = neg;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s[0][t] += s[0][t + w];
+            s[1][t] += s[1][t + w];
+            s[2][t] += s[2][t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[3 * pl] = s[0][0];
+        out[3 * pl + 1] = s[1][0];
+        out[3 * pl + 2] = s[2][0];
+    }
+}
+
+hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s) {
+    if (a.npts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(finalize_kernel, dim3((a.npts + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_plane_reduce(const double *contrib, const long long *offs, int nplanes,
+                               double *out, hipStream_t s) {
+    if (nplanes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(plane_reduce_kernel, dim3(nplanes), dim3(256), 0, s, contrib, offs, out);
+    return hipGetLastError();
+}
+
+}  // namespace gimic

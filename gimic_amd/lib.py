@@ -1,0 +1,271 @@
+"""ctypes binding of libgimic_hip.so (include/gimic_hip.h).
+
+The library is the only compute path: importing this module fails loudly
+if the HIP library has not been built (python -m gimic_amd.build).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libgimic_hip.so")
+
+SPIN = {"total": 0, "alpha": 1, "beta": 2, "spindens": 3}
+PREC = {"fp64": 0, "bf16x3": 1, "mixed": 1}
+
+EXPORTS = [
+    "gimic_init", "gimic_finalize", "gimic_set_uhf", "gimic_set_magnet", "gimic_set_spin",
+    "gimic_set_screening", "gimic_calc_jtensor", "gimic_calc_jvector", "gimic_calc_modj",
+    "gimic_get_gauss_points", "mkgausspoints", "gimic_last_error", "gimic_ctx_create_files",
+    "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
+    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
+    "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_sph_transform", "gimic_quadrature",
+    "gimic_selftest_mfma_layout",
+]
+
+_lib = None
+
+
+class GimicError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the library (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise GimicError(f"{LIBPATH} not found: build it with `python -m gimic_amd.build`")
+    L = ctypes.CDLL(LIBPATH)
+    P, I, D, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
+    L.gimic_last_error.restype = ctypes.c_char_p
+    sig = {
+        "gimic_init": [ctypes.c_char_p, ctypes.c_char_p],
+        "gimic_finalize": [],
+        "gimic_set_uhf": [P],
+        "gimic_set_magnet": [P],
+        "gimic_set_spin": [ctypes.c_char_p],
+        "gimic_set_screening": [P],
+        "gimic_calc_jtensor": [P, P],
+        "gimic_calc_jvector": [P, P],
+        "gimic_calc_modj": [P, P],
+        "gimic_get_gauss_points": [P, P, P, P, P, P],
+        "mkgausspoints": [P, P, P, P, P, P],
+        "gimic_ctx_create_files": [P, ctypes.c_char_p, ctypes.c_char_p, I, D],
+        "gimic_ctx_create": [P, I, P, I, P, P, P, P, P, P, I, I, D],
+        "gimic_ctx_set_density": [P, I, P],
+        "gimic_ctx_set_flags": [P, I, I, I],
+        "gimic_ctx_info": [P, P, P, P, P, P],
+        "gimic_ctx_destroy": [P],
+        "gimic_ctx_atoms": [P, P, P],
+        "gimic_ctx_set_timing": [P, I],
+        "gimic_ctx_get_timing": [P, P, P],
+        "gimic_jtensor_batch": [P, I64, P, P, I, I],
+        "gimic_jtensor_batch_device": [P, I64, P, P, I, I, P],
+        "gimic_jtensor_multi_device": [P, I64, P, P, I, P, I, P],
+        "gimic_flux_planes": [P, I, P, P, P, P, P, P, P, I, I, P, P, P],
+        "gimic_sph_transform": [I, P],
+        "gimic_quadrature": [D, D, I, I, I, P, P],
+        "gimic_selftest_mfma_layout": [P, P],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        if name.startswith("gimic_ctx") or name in ("gimic_jtensor_batch", "gimic_jtensor_batch_device",
+                                                    "gimic_jtensor_multi_device", "gimic_flux_planes",
+                                                    "gimic_sph_transform", "gimic_quadrature",
+                                                    "gimic_selftest_mfma_layout"):
+            fn.restype = I
+        else:
+            fn.restype = None
+    _lib = L
+    return L
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().gimic_last_error().decode(errors="replace")
+        raise GimicError(f"gimic error {rc}: {msg}")
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    return a if shape is None else a.reshape(shape)
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def sph_transform(l):
+    nc = (l + 1) * (l + 2) // 2
+    T = np.zeros((2 * l + 1, nc))
+    check(load().gimic_sph_transform(l, ptr(T)))
+    return T
+
+
+def quadrature(a, b, order, npts, kind="gauss"):
+    """Composite nodes/weights as the reference's setup_gauss_data."""
+    pts = np.zeros(npts)
+    w = np.zeros(npts)
+    check(load().gimic_quadrature(float(a), float(b), int(order), int(npts),
+                                  {"gauss": 0, "lobatto": 1}[kind], ptr(pts), ptr(w)))
+    return pts, w
+
+
+class Context:
+    """Handle of one basis + density family resident on the current GPU."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self._lib = load()
+
+    # -- construction ----------------------------------------------------
+    @classmethod
+    def from_files(cls, molfile, densfile=None, *, uhf=False, screening=1e-8):
+        L = load()
+        h = ctypes.c_void_p()
+        check(L.gimic_ctx_create_files(ctypes.byref(h), os.fsencode(molfile),
+                                       os.fsencode(densfile) if densfile else None,
+                                       int(bool(uhf)), float(screening if screening else -1.0)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, coords, shell_atom, shell_l, shell_nprim, shell_poff, exps, coefs, *,
+                    turbomole=False, spherical=False, screening=1e-8):
+        L = load()
+        h = ctypes.c_void_p()
+        coords = _f64(coords, (-1, 3))
+        sa, sl, sn, so = _i32(shell_atom), _i32(shell_l), _i32(shell_nprim), _i32(shell_poff)
+        ex, co = _f64(exps), _f64(coefs)
+        check(L.gimic_ctx_create(ctypes.byref(h), coords.shape[0], ptr(coords), len(sl), ptr(sa),
+                                 ptr(sl), ptr(sn), ptr(so), ptr(ex), ptr(co), int(bool(turbomole)),
+                                 int(bool(spherical)), float(screening if screening else -1.0)))
+        return cls(h)
+
+    def close(self):
+        if self._h is not None:
+            check(self._lib.gimic_ctx_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state -------------------------------------------------------------
+    def info(self):
+        v = [ctypes.c_int() for _ in range(5)]
+        check(self._lib.gimic_ctx_info(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["nbf", "nbf_cart", "natoms", "nshell", "nset"], [x.value for x in v]))
+
+    @property
+    def nbf(self):
+        return self.info()["nbf"]
+
+    def set_timing(self, on=True):
+        check(self._lib.gimic_ctx_set_timing(self._h, int(bool(on))))
+
+    def get_timing(self):
+        """{kernel: (total ms, launches)} for basis_eval, contract, finalize"""
+        ms = np.zeros(3)
+        n = np.zeros(3, dtype=np.int64)
+        check(self._lib.gimic_ctx_get_timing(self._h, ptr(ms), ptr(n)))
+        return {k: (float(ms[i]), int(n[i]))
+                for i, k in enumerate(["basis_eval", "contract", "finalize"])}
+
+    def atoms(self):
+        """(coords (natoms,3) bohr, charges (natoms,))"""
+        n = self.info()["natoms"]
+        c = np.zeros((n, 3))
+        q = np.zeros(n)
+        check(self._lib.gimic_ctx_atoms(self._h, ptr(c), ptr(q)))
+        return c, q
+
+    def set_density(self, dens):
+        """dens: (nset, 4, n, n) with [.., mu, nu] = D(mu, nu)."""
+        d = np.asarray(dens, dtype=np.float64)
+        if d.ndim == 3:
+            d = d[None]
+        nset = d.shape[0]
+        flat = np.ascontiguousarray(d.transpose(0, 1, 3, 2)).reshape(-1)
+        check(self._lib.gimic_ctx_set_density(self._h, nset, ptr(flat)))
+
+    def set_flags(self, giao=True, diamag=True, paramag=True):
+        check(self._lib.gimic_ctx_set_flags(self._h, int(giao), int(diamag), int(paramag)))
+
+    # -- evaluation --------------------------------------------------------
+    def jtensor(self, points, spin="total", precision="fp64"):
+        """(N,3) host points -> (N,9) tensors, element m + 3b = J(m,b)."""
+        pts = _f64(points, (-1, 3))
+        out = np.zeros((pts.shape[0], 9))
+        check(self._lib.gimic_jtensor_batch(self._h, pts.shape[0], ptr(pts), ptr(out),
+                                            SPIN[spin], PREC[precision]))
+        return out
+
+    def jtensor_device(self, points, spin="total", precision="fp64", out=None, stream=None):
+        """torch float64 (N,3) tensor on the GPU -> (N,9) tensor (asynchronous)."""
+        import torch
+        pts = points.contiguous()
+        assert pts.dtype == torch.float64 and pts.is_cuda
+        n = pts.shape[0]
+        if out is None:
+            out = torch.empty((n, 9), dtype=torch.float64, device=pts.device)
+        s = ctypes.c_void_p(stream) if stream is not None else \
+            ctypes.c_void_p(torch.cuda.current_stream(pts.device).cuda_stream)
+        check(self._lib.gimic_jtensor_batch_device(self._h, n, ctypes.c_void_p(pts.data_ptr()),
+                                                   ctypes.c_void_p(out.data_ptr()), SPIN[spin],
+                                                   PREC[precision], s))
+        return out
+
+    def jtensor_multi_device(self, points, spins, precision="fp64", out=None, stream=None):
+        import torch
+        pts = points.contiguous()
+        n = pts.shape[0]
+        sp = _i32([SPIN[s] for s in spins])
+        if out is None:
+            out = torch.empty((len(spins), n, 9), dtype=torch.float64, device=pts.device)
+        s = ctypes.c_void_p(stream) if stream is not None else \
+            ctypes.c_void_p(torch.cuda.current_stream(pts.device).cuda_stream)
+        check(self._lib.gimic_jtensor_multi_device(self._h, n, ctypes.c_void_p(pts.data_ptr()),
+                                                   ctypes.c_void_p(out.data_ptr()), len(spins),
+                                                   ptr(sp), PREC[precision], s))
+        return out
+
+    def flux_planes(self, planes, spin="total", precision="fp64", return_jt=False,
+                    modulus=False):
+        """planes: list of dicts with points (N,3), weights (N), normal, bfield,
+        center, radius.  Returns (nplanes, 3): net, positive, negative (au);
+        with modulus=True also the signed-modulus sums (nplanes, 3)."""
+        offs = np.zeros(len(planes) + 1, dtype=np.int64)
+        for k, p in enumerate(planes):
+            offs[k + 1] = offs[k] + len(p["weights"])
+        r = _f64(np.concatenate([np.asarray(p["points"]).reshape(-1, 3) for p in planes]))
+        w = _f64(np.concatenate([np.asarray(p["weights"]).reshape(-1) for p in planes]))
+        nrm = _f64([p["normal"] for p in planes])
+        bf = _f64([p["bfield"] for p in planes])
+        ctr = _f64([p["center"] for p in planes])
+        rad = _f64([p.get("radius", 1e10) for p in planes])
+        out = np.zeros((len(planes), 3))
+        mod = np.zeros((len(planes), 3)) if modulus else None
+        jt = np.zeros((int(offs[-1]), 9)) if return_jt else None
+        check(self._lib.gimic_flux_planes(self._h, len(planes), ptr(offs), ptr(r), ptr(w),
+                                          ptr(nrm), ptr(bf), ptr(ctr), ptr(rad), SPIN[spin],
+                                          PREC[precision], ptr(out),
+                                          ptr(mod) if mod is not None else None,
+                                          ptr(jt) if jt is not None else None))
+        res = (out,) + ((mod,) if modulus else ()) + ((jt,) if return_jt else ())
+        return res if len(res) > 1 else out

@@ -1,0 +1,220 @@
+"""Seeded synthetic molecules, bases and density families for the workloads.
+
+All coordinates are bohr.  Bases are per element: each shell draws its
+primitive count uniformly from 1..6, exponents log-uniform in [0.1, 1e4]
+and raw coefficients from N(0, 1); the library normalises them the way the
+reference does (basis.f90:164-191).  Densities: C = Q of the QR of an
+N(0,1) nbf x nbf matrix, D = 2 C_occ C_occ^T, DB_k = C_occ X_k C_virt^T
+minus its transpose with X_k ~ N(0, 0.1^2).  Seeds: 20261202 + config id.
+
+Workloads:
+  ring12   12-atom ring: 6 C on a hexagon (1.39 A bonds) + 6 H radially at
+           1.09 A; C 3s2p1d, H 2s1p spherical; nbf 114 (120 cartesian); nocc 21
+  cage60   60 C on a truncated icosahedron of radius 6.7 bohr; 4s3p2d1f
+           spherical per atom; nbf 1800 (2100 cartesian); nocc 180
+  sheet30  tetracene C18H12 (1.40 A C-C bonds, 1.09 A C-H), 3s2p1d / 2s1p
+           spherical; nbf 312; nocc 60 (30 atoms cannot reach 450 functions
+           with these shells: 30 carbons would give 420)
+  sheet200 H-capped hexagonal carbon sheet C164H36 (200 atoms), 3s2p1d / 2s1p
+           spherical, nbf 2476; separate alpha/beta densities, nocc 610/600
+"""
+import math
+
+import numpy as np
+
+ANGSTROM = 1.889725949  # bohr per angstrom (globals.f90:53)
+SEED0 = 20261202
+
+SHELLS = {"3s2p1d": [0, 0, 0, 1, 1, 2], "2s1p": [0, 0, 1],
+          "4s3p2d1f": [0, 0, 0, 0, 1, 1, 1, 2, 2, 3]}
+
+
+class System:
+    """Molecule + basis (segmented shells in function order) + densities."""
+
+    def __init__(self, name):
+        self.name = name
+        self.symbols, self.charges, self.coords = [], [], []
+        self.shell_atom, self.shell_l, self.shell_nprim, self.shell_poff = [], [], [], []
+        self.exps, self.coefs = [], []
+        self.spherical = True
+        self.dens = None  # (nset, 4, n, n), [mu, nu] = D(mu, nu)
+
+    @property
+    def natoms(self):
+        return len(self.symbols)
+
+    @property
+    def nbf(self):
+        return sum(2 * l + 1 if self.spherical else (l + 1) * (l + 2) // 2 for l in self.shell_l)
+
+    @property
+    def nbf_cart(self):
+        return sum((l + 1) * (l + 2) // 2 for l in self.shell_l)
+
+    def arrays(self):
+        return dict(coords=np.asarray(self.coords, dtype=np.float64).reshape(-1, 3),
+                    shell_atom=np.asarray(self.shell_atom, dtype=np.int32),
+                    shell_l=np.asarray(self.shell_l, dtype=np.int32),
+                    shell_nprim=np.asarray(self.shell_nprim, dtype=np.int32),
+                    shell_poff=np.asarray(self.shell_poff, dtype=np.int32),
+                    exps=np.asarray(self.exps, dtype=np.float64),
+                    coefs=np.asarray(self.coefs, dtype=np.float64), turbomole=0)
+
+    def context(self, screening=1e-8):
+        from .lib import Context
+        a = self.arrays()
+        ctx = Context.from_arrays(a["coords"], a["shell_atom"], a["shell_l"], a["shell_nprim"],
+                                  a["shell_poff"], a["exps"], a["coefs"],
+                                  spherical=self.spherical, screening=screening)
+        if self.dens is not None:
+            ctx.set_density(self.dens)
+        return ctx
+
+
+def element_basis(rng, shape):
+    out = []
+    for l in SHELLS[shape]:
+        n = int(rng.integers(1, 7))
+        ex = np.exp(rng.uniform(math.log(0.1), math.log(1.0e4), n))
+        co = rng.standard_normal(n)
+        out.append((l, ex, co))
+    return out
+
+
+def build_system(name, symbols, coords, shapes, seed, spherical=True):
+    rng = np.random.default_rng(seed)
+    bases = {el: element_basis(rng, shp) for el, shp in shapes.items()}
+    s = System(name)
+    s.spherical = spherical
+    for a, (el, xyz) in enumerate(zip(symbols, coords)):
+        s.symbols.append(el)
+        s.charges.append({"H": 1.0, "C": 6.0}.get(el, 0.0))
+        s.coords.append([float(v) for v in xyz])
+        for l, ex, co in bases[el]:
+            s.shell_atom.append(a)
+            s.shell_l.append(l)
+            s.shell_nprim.append(len(ex))
+            s.shell_poff.append(len(s.exps))
+            s.exps.extend(ex.tolist())
+            s.coefs.extend(co.tolist())
+    return s, rng
+
+
+def density_family(rng, n, nocc, sigma=0.1):
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    co, cv = q[:, :nocc], q[:, nocc:]
+    d = np.empty((4, n, n))
+    d[0] = 2.0 * co @ co.T
+    for k in range(3):
+        x = rng.normal(0.0, sigma, (nocc, n - nocc))
+        m = co @ x @ cv.T
+        d[1 + k] = m - m.T
+    return d
+
+
+def ring12(spherical=True):
+    r_cc = 1.39 * ANGSTROM
+    r_ch = 1.09 * ANGSTROM
+    sym, xyz = [], []
+    for k in range(6):
+        t = k * math.pi / 3.0
+        sym.append("C")
+        xyz.append([r_cc * math.cos(t), r_cc * math.sin(t), 0.0])
+    for k in range(6):
+        t = k * math.pi / 3.0
+        sym.append("H")
+        xyz.append([(r_cc + r_ch) * math.cos(t), (r_cc + r_ch) * math.sin(t), 0.0])
+    s, rng = build_system("ring12", sym, xyz, {"C": "3s2p1d", "H": "2s1p"}, SEED0 + 1, spherical)
+    s.dens = density_family(rng, s.nbf, 21)[None]
+    return s
+
+
+def truncated_icosahedron(radius):
+    phi = (1.0 + math.sqrt(5.0)) / 2.0
+    base = [(0.0, 1.0, 3 * phi), (1.0, 2 + phi, 2 * phi), (phi, 2.0, 2 * phi + 1)]
+    pts = set()
+    for v in base:
+        for sx in (1, -1):
+            for sy in (1, -1):
+                for sz in (1, -1):
+                    w = (v[0] * sx, v[1] * sy, v[2] * sz)
+                    for p in ((w[0], w[1], w[2]), (w[1], w[2], w[0]), (w[2], w[0], w[1])):
+                        pts.add(tuple(round(c, 12) for c in p))
+    pts = np.array(sorted(pts))
+    assert pts.shape == (60, 3)
+    return pts * (radius / np.linalg.norm(pts[0]))
+
+
+def cage60(spherical=True, nocc=180):
+    xyz = truncated_icosahedron(6.7)
+    s, rng = build_system("cage60", ["C"] * 60, xyz, {"C": "4s3p2d1f"}, SEED0 + 2, spherical)
+    s.dens = density_family(rng, s.nbf, nocc)[None]
+    return s
+
+
+def _hex_sheet(ring_centres, cc):
+    """carbon sites of fused hexagons (pointy-top) + H caps on 2-coordinated C"""
+    carbons = []
+
+    def add(p):
+        for q in carbons:
+            if np.linalg.norm(q - p) < 1e-6:
+                return
+        carbons.append(p)
+
+    for cx, cy in ring_centres:
+        for k in range(6):
+            t = math.pi / 6.0 + k * math.pi / 3.0
+            add(np.array([cx + cc * math.cos(t), cy + cc * math.sin(t), 0.0]))
+    carbons = np.array(carbons)
+    return carbons
+
+
+def _cap(carbons, cc, ch):
+    hyd = []
+    for i, p in enumerate(carbons):
+        nb = [q for j, q in enumerate(carbons) if j != i and np.linalg.norm(q - p) < 1.1 * cc]
+        if len(nb) == 2:
+            v = 2 * p - nb[0] - nb[1]
+            hyd.append(p + ch * v / np.linalg.norm(v))
+    return np.array(hyd)
+
+
+def sheet30(spherical=True):
+    cc, ch = 1.40 * ANGSTROM, 1.09 * ANGSTROM
+    w = math.sqrt(3.0) * cc
+    centres = [(k * w, 0.0) for k in range(4)]  # tetracene: 4 linearly fused rings
+    c = _hex_sheet(centres, cc)
+    h = _cap(c, cc, ch)
+    sym = ["C"] * len(c) + ["H"] * len(h)
+    xyz = np.vstack([c, h])
+    s, rng = build_system("sheet30", sym, xyz, {"C": "3s2p1d", "H": "2s1p"}, SEED0 + 3, spherical)
+    s.dens = density_family(rng, s.nbf, 60)[None]
+    return s
+
+
+def sheet200(spherical=True):
+    cc, ch = 1.42 * ANGSTROM, 1.09 * ANGSTROM
+    w = math.sqrt(3.0) * cc
+    centres = []
+    for row in range(7):
+        for col in range(10):
+            centres.append((col * w + (row % 2) * w / 2.0, row * 1.5 * cc))
+    c = _hex_sheet(centres, cc)
+    h = _cap(c, cc, ch)
+    sym = ["C"] * len(c) + ["H"] * len(h)
+    xyz = np.vstack([c, h])
+    xyz -= xyz.mean(axis=0)
+    s, rng = build_system("sheet200", sym, xyz, {"C": "3s2p1d", "H": "2s1p"}, SEED0 + 5, spherical)
+    da = density_family(rng, s.nbf, 610)
+    db = density_family(rng, s.nbf, 600)
+    s.dens = np.stack([da, db])
+    return s
+
+
+def cube_grid(n, spacing, origin):
+    """n^3 points, x fastest (grid.f90:478-495 index order)"""
+    ax = origin + spacing * np.arange(n)
+    z, y, x = np.meshgrid(ax, ax, ax, indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)

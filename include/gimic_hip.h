@@ -1,0 +1,178 @@
+/*
+ * gimic_hip.h - C ABI of the MI355X (gfx950) GIMIC current-density library.
+ *
+ * Part 1 is the reference's own C interface, same names, same by-pointer
+ * calling convention and the same tensor layout jt[m + 3*b] = J(m,b)
+ * (m = current component, b = field component).  Part 2 is a batched,
+ * handle-based interface that keeps the density family resident in HBM and
+ * evaluates many points per call.  Plain pointers and sizes only.
+ */
+#ifndef GIMIC_HIP_H
+#define GIMIC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Part 1: drop-in replacements (one implicit global context per process).
+ * Errors are reported on stderr and through gimic_last_error(); the library
+ * never terminates the process.
+ * ------------------------------------------------------------------------- */
+
+/* src/libgimic/gimic_interface.h:10, gimic_interface.f90:26-70.
+ * Reads a MOL/INTGRL basis and an XDENS density file; cartesian basis,
+ * screening threshold 1e-6, GIAO/diamagnetic/paramagnetic terms on, B = 0,
+ * spin case "total". */
+void gimic_init(const char *molfile, const char *densfile);
+/* gimic_interface.h:11, gimic_interface.f90:72-78 */
+void gimic_finalize(void);
+/* gimic_interface.h:12, gimic_interface.f90:80-86.  Takes effect at the next
+ * gimic_init (the reference's gimic_init resets it to closed shell). */
+void gimic_set_uhf(int *uhf);
+/* gimic_interface.h:13, gimic_interface.f90:88-91 */
+void gimic_set_magnet(const double *b);
+/* gimic_interface.h:14, gimic_interface.f90:93-112: "alpha", "beta",
+ * "total", "spindens" */
+void gimic_set_spin(const char *s);
+/* gimic_interface.h:15, gimic_interface.f90:114-117.  As in the reference the
+ * threshold is applied when the basis is set up, i.e. at the next gimic_init. */
+void gimic_set_screening(const double *thrs);
+/* gimic_interface.h:16, gimic_interface.f90:119-139 */
+void gimic_calc_jtensor(const double *r, double *jt);
+/* gimic_interface.h:17, gimic_interface.f90:141-151: jv = J B */
+void gimic_calc_jvector(const double *r, double *jv);
+/* gimic_interface.h:18, gimic_interface.f90:153-163 (the reference stops with
+ * "NOT IMPLEMENTED YET"); here |J B|. */
+void gimic_calc_modj(const double *r, double *mj);
+/* src/libgimic/gausspoints.h:10, gausspoints.f90:12-29: composite
+ * Gauss-Legendre nodes/weights on [a,b], npts points in blocks of order */
+void gimic_get_gauss_points(double *a, double *b, int *npts, int *order, double *pts,
+                            double *wgts);
+/* gausspoints.h:11, gausspoints.c:4-7 */
+void mkgausspoints(double *a, double *b, int *npts, int *order, double *pts, double *wgts);
+
+/* ---------------------------------------------------------------------------
+ * Part 2: batched handle API (no counterpart in the reference's C ABI; it
+ * replaces the per-point ctensor calls of the reference's loops
+ * src/fgimic/jfield.f90:114-129 and src/fgimic/integral.f90:107-165).
+ * All functions return 0 on success or a negative GIMIC_E* code.
+ * ------------------------------------------------------------------------- */
+
+typedef struct gimic_ctx gimic_ctx;
+
+enum {
+    GIMIC_OK = 0,
+    GIMIC_EINVAL = -1,  /* bad argument */
+    GIMIC_EIO = -2,     /* file missing or malformed */
+    GIMIC_EDEVICE = -3, /* HIP runtime error or no device */
+    GIMIC_ENOMEM = -4,  /* device allocation failed */
+    GIMIC_ESTATE = -5   /* e.g. no density set, beta requested for closed shell */
+};
+
+enum { GIMIC_SPIN_TOTAL = 0, GIMIC_SPIN_ALPHA = 1, GIMIC_SPIN_BETA = 2, GIMIC_SPIN_SPINDENS = 3 };
+
+/* contraction precision: FP64 MFMA, or split-BF16 (hi*hi + hi*lo + lo*hi)
+ * MFMA with FP32 accumulation; basis values, partial sums across tiles, the
+ * tensor assembly and flux sums stay FP64 in both */
+enum { GIMIC_PREC_FP64 = 0, GIMIC_PREC_BF16X3 = 1 };
+
+/* message of the last failing call on this thread ("" if none) */
+const char *gimic_last_error(void);
+
+/* Basis from a MOL/INTGRL file (src/libgimic/intgrl.f90:20-264), density
+ * from an XDENS file (dens.f90:56-135; UHF perturbed densities halved and
+ * TURBOMOLE order permuted as the reference reader does).  densfile may be
+ * NULL to set densities later.  screen_thrs <= 0 disables screening. */
+int gimic_ctx_create_files(gimic_ctx **ctx, const char *molfile, const char *densfile, int uhf,
+                           double screen_thrs);
+
+/* Basis from arrays.  Shells (segmented contractions) in function order:
+ * atom-major, atom indices 0-based and non-decreasing; shell s has angular
+ * momentum shell_l[s] <= 5 and shell_nprim[s] primitives at
+ * exps/coefs[shell_poff[s] ...] (raw coefficients, normalised here as the
+ * reference's norm_ctr does).  coords: 3*natoms bohr.  turbomole selects the
+ * TURBOMOLE cartesian component order; spherical selects real solid
+ * harmonics (2l+1 functions per shell, see gimic_sph_transform). */
+int gimic_ctx_create(gimic_ctx **ctx, int natoms, const double *coords, int nshell,
+                     const int *shell_atom, const int *shell_l, const int *shell_nprim,
+                     const int *shell_poff, const double *exps, const double *coefs,
+                     int turbomole, int spherical, double screen_thrs);
+
+/* nset = 1 (closed shell) or 2 (alpha, beta): nset*4 blocks of nbf*nbf
+ * column-major doubles D, DBx, DBy, DBz, used exactly as given. */
+int gimic_ctx_set_density(gimic_ctx *ctx, int nset, const double *dens);
+
+/* GIAO, diamagnetic and paramagnetic terms (jtensor.F90:180-228) */
+int gimic_ctx_set_flags(gimic_ctx *ctx, int giao, int diamag, int paramag);
+
+int gimic_ctx_info(const gimic_ctx *ctx, int *nbf, int *nbf_cart, int *natoms, int *nshell,
+                   int *nset);
+
+/* atom coordinates (3*natoms, bohr) and nuclear charges as read
+ * (intgrl.f90:91-115); either pointer may be NULL */
+int gimic_ctx_atoms(const gimic_ctx *ctx, double *coords, double *charges);
+
+/* Per-kernel timing (no counterpart in the reference): when on, every
+ * launch of the basis-evaluation, contraction and finalize kernels is
+ * bracketed by HIP events on its stream; gimic_ctx_get_timing waits for
+ * them and returns the accumulated milliseconds and launch counts (3 each,
+ * in that order).  Turning timing on or off resets the totals. */
+int gimic_ctx_set_timing(gimic_ctx *ctx, int on);
+int gimic_ctx_get_timing(gimic_ctx *ctx, double *ms, long long *launches);
+
+int gimic_ctx_destroy(gimic_ctx *ctx);
+
+/* J tensors at npts points: r[3*i..] (bohr) -> jt[9*i + m + 3*b].
+ * Host pointers; the call returns when the results are in jt. */
+int gimic_jtensor_batch(gimic_ctx *ctx, int64_t npts, const double *r, double *jt, int spin,
+                        int precision);
+
+/* Same with device pointers (HBM) on the given HIP stream (NULL: the
+ * context's stream); asynchronous with respect to the host. */
+int gimic_jtensor_batch_device(gimic_ctx *ctx, int64_t npts, const double *d_r, double *d_jt,
+                               int spin, int precision, void *stream);
+
+/* Several spin cases from one pass over the basis values: jt holds nspin
+ * blocks of 9*npts doubles in the order of spins[].  Device pointers. */
+int gimic_jtensor_multi_device(gimic_ctx *ctx, int64_t npts, const double *d_r, double *d_jt,
+                               int nspin, const int *spins, int precision, void *stream);
+
+/* Plane flux (src/fgimic/integral.f90:50-186) for nplanes planes in one call.
+ * Points of plane k are r[3*i..] for plane_offs[k] <= i < plane_offs[k+1]
+ * with product quadrature weights w[i]; per plane: unit normal, B field,
+ * centre and radius bound (points farther than radius from the centre
+ * contribute zero; pass 1e10 for none).  out[3*k..] = net, positive,
+ * negative flux in au.  out_mod (optional, may be NULL) receives the same
+ * three sums of the signed modulus of J B (integral.f90:190-324) from the
+ * same tensors.  Host pointers; jt (optional, may be NULL) receives the
+ * tensors of all points (9 per point). */
+int gimic_flux_planes(gimic_ctx *ctx, int nplanes, const int64_t *plane_offs, const double *r,
+                      const double *w, const double *normals, const double *bfields,
+                      const double *centers, const double *radii, int spin, int precision,
+                      double *out, double *out_mod, double *jt);
+
+/* Real solid-harmonic transform for angular momentum l <= 5:
+ * T[(m+l)*ncart + c], m = -l..l, cartesian components in the reference's
+ * non-TURBOMOLE order (gtodefs.f90:86-106); for l <= 1 the identity (s, and
+ * p as x, y, z).  Rows have unit norm under the reference normalisation. */
+int gimic_sph_transform(int l, double *T);
+
+/* Composite quadrature on [a,b] as the reference's setup_gauss_data
+ * (gaussint.f90:267-319): kind 0 Gauss-Legendre, 1 Gauss-Lobatto. */
+int gimic_quadrature(double a, double b, int order, int npts, int kind, double *pts,
+                     double *wgts);
+
+/* Diagnostic (no counterpart in the reference): runs the two MFMA builtins
+ * the kernels use on one wavefront with lane-numbered operands and returns
+ * 512 values each (rows then columns; see gimic_amd/csrc/selftest.hip), so a
+ * test can confirm the lane layout the kernels assume. */
+int gimic_selftest_mfma_layout(double *f64, float *bf16);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GIMIC_HIP_H */

@@ -1,0 +1,55 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+REFERENCE = "/root/reference"
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+def golden(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def have_reference_build():
+    from oracle import refrun
+    return refrun.available() and os.path.isdir(REFERENCE)
+
+
+def rel_err(a, b, floor_frac=1e-3):
+    """per-point ||a-b|| / max(||b||, floor_frac * max ||b||) over (N,9) tensors"""
+    a = np.asarray(a).reshape(len(a), -1)
+    b = np.asarray(b).reshape(len(b), -1)
+    nb = np.linalg.norm(b, axis=1)
+    den = np.maximum(nb, floor_frac * nb.max())
+    return np.linalg.norm(a - b, axis=1) / den
+
+
+def write_fixture(tmpdir, npz):
+    """MOL and XDENS text files from a golden npz (values as the reference stored them)"""
+    d = golden(npz)
+    mol = os.path.join(tmpdir, "MOL")
+    xd = os.path.join(tmpdir, "XDENS")
+    with open(mol, "w") as f:
+        f.write(str(d["mol"]))
+    np.savetxt(xd, d["xdens"], fmt="%.17e")
+    return mol, xd
+
+
+@pytest.fixture(scope="session")
+def c4h4_files(tmp_path_factory):
+    return write_fixture(str(tmp_path_factory.mktemp("c4h4")), "c4h4.npz")
+
+
+@pytest.fixture(scope="session")
+def openshell_files(tmp_path_factory):
+    return write_fixture(str(tmp_path_factory.mktemp("openshell")), "openshell.npz")

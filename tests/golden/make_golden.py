@@ -1,0 +1,136 @@
+"""Write the golden fixtures under tests/golden/ (run in a checkout that has
+the reference tree; the GPU box only reads the files written here).
+
+  c4h4.npz        test/c4h4/{MOL,XDENS} (basis text + density values as
+                  stored), read-grid points and reference jvec.vtu vectors,
+                  integration/reference/stdout flux numbers
+  openshell.npz   test/open-shell/{MOL,XDENS}, integration and 3d reference
+                  outputs (stdout numbers, jvec*.vti vectors)
+  ring12_ref.npz  flang-built reference (oracle/_ref) tensors, all nine
+                  components, for the synthetic 12-atom ring at 256 seeded
+                  points, the cartesian-equivalent density it was fed, and
+                  composite Gauss/Lobatto rules from the reference's
+                  setup_gauss_data
+
+Usage: python tests/golden/make_golden.py [/root/reference]
+"""
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+NUM = re.compile(r"[-+]?(?:\d+\.\d*|\.\d+|\d+)(?:[eEdD][-+]?\d+)?")
+
+
+def numbers(text):
+    return [float(x.replace("D", "E").replace("d", "e")) for x in NUM.findall(text)]
+
+
+def xdens_values(path):
+    vals = []
+    with open(path) as f:
+        for line in f:
+            s = line.split()
+            if s:
+                vals.append(float(s[0].replace("D", "E")))
+    return np.array(vals)
+
+
+def vtu_vectors(path, npts):
+    lines = open(path).read().splitlines()
+    i = next(k for k, l in enumerate(lines) if 'Name="vectors"' in l)
+    return np.array([[float(x) for x in lines[i + 1 + k].split()] for k in range(npts)])
+
+
+def vti_data(path):
+    txt = open(path).read()
+    head = txt[:txt.index("<PointData")]
+    extent = [int(x) for x in re.search(r'WholeExtent="([^"]+)"', head).group(1).split()]
+    origin = [float(x) for x in re.search(r'Origin="([^"]+)"', head).group(1).split()]
+    spacing = [float(x) for x in re.search(r'Spacing="([^"]+)"', head).group(1).split()]
+    body = txt[txt.index("<DataArray"):]
+    body = body[body.index(">") + 1:body.index("</DataArray>")]
+    return np.array(extent), np.array(origin), np.array(spacing), np.array(numbers(body))
+
+
+def flux_block(stdout, marker):
+    """numbers of the 10-line block starting at `marker` (runtest filter)."""
+    lines = stdout.splitlines()
+    i = next(k for k, l in enumerate(lines) if marker in l)
+    return numbers("\n".join(lines[i:i + 10]))
+
+
+def main(ref):
+    t = os.path.join(ref, "test")
+    # ---- c4h4 ----
+    d = os.path.join(t, "c4h4")
+    grid = np.loadtxt(os.path.join(d, "read-grid", "gridfile.grd"))
+    out = dict(
+        mol=np.array(open(os.path.join(d, "MOL")).read()),
+        xdens=xdens_values(os.path.join(d, "XDENS")),
+        readgrid_points=grid,
+        readgrid_jvec=vtu_vectors(os.path.join(d, "read-grid", "reference", "jvec.vtu"), len(grid)),
+        integration_inp=np.array(open(os.path.join(d, "integration", "gimic.inp")).read()),
+        integration_stdout=np.array(open(os.path.join(d, "integration", "reference",
+                                                      "stdout")).read()),
+    )
+    np.savez_compressed(os.path.join(HERE, "c4h4.npz"), **out)
+    # ---- open-shell ----
+    d = os.path.join(t, "open-shell")
+    out = dict(
+        mol=np.array(open(os.path.join(d, "MOL")).read()),
+        xdens=xdens_values(os.path.join(d, "XDENS")),
+        integration_inp=np.array(open(os.path.join(d, "integration", "gimic.inp")).read()),
+        integration_stdout=np.array(open(os.path.join(d, "integration", "reference",
+                                                      "stdout")).read()),
+        threed_inp=np.array(open(os.path.join(d, "3d", "gimic.inp")).read()),
+    )
+    for name in ["jvec", "jvecalpha", "jvecbeta", "jvecspindens"]:
+        ext, org, spc, vals = vti_data(os.path.join(d, "3d", "reference", name + ".vti"))
+        out[name] = vals.astype(np.float32)  # printed with 6 digits (e14.6)
+        out["vti_extent"], out["vti_origin"], out["vti_spacing"] = ext, org, spc
+    np.savez_compressed(os.path.join(HERE, "openshell.npz"), **out)
+    # ---- synthetic ring through the flang-built reference ----
+    from gimic_amd import synth
+    from oracle import molio, refrun, sph
+    import tempfile
+    s = synth.ring12()
+    rng = np.random.default_rng(7)
+    pts = rng.uniform(-6.0, 6.0, (256, 3))
+    pts[:, 2] *= 0.5
+    dcart = sph.to_cartesian(s.dens, s.shell_l)
+    b = molio.Basis()
+    b.symbols, b.charges, b.coords = s.symbols, s.charges, s.coords
+    b.shell_atom, b.shell_l, b.shell_nprim, b.shell_poff = (s.shell_atom, s.shell_l,
+                                                             s.shell_nprim, s.shell_poff)
+    b.exps, b.coefs = s.exps, s.coefs
+    with tempfile.TemporaryDirectory() as td:
+        mol = os.path.join(td, "MOL")
+        molio.write_mol(mol, b)
+        res = {}
+        for flags in [(1, 1, 1), (0, 1, 1), (1, 0, 1), (1, 1, 0)]:
+            key = "jt_%d%d%d" % flags
+            res[key] = refrun.ref_jtensor(mol, pts, dens=molio.pack_dens(dcart), screening=1e-8,
+                                          giao=flags[0], diamag=flags[1], paramag=flags[2])
+        res["jt_noscreen"] = refrun.ref_jtensor(mol, pts, dens=molio.pack_dens(dcart),
+                                                screening=-1)
+        molread = molio.read_mol(mol)
+        assert molread.nbf == s.nbf_cart
+    rules = {}
+    for kind in ("gauss", "lobatto"):
+        for order in (3, 5, 7, 9, 11):
+            p, w = refrun.ref_gauss_points(0.0, 7.3, order, 4 * order, kind)
+            rules[f"{kind}_{order}_pts"] = p
+            rules[f"{kind}_{order}_wgt"] = w
+    np.savez_compressed(os.path.join(HERE, "ring12_ref.npz"), points=pts, dens_cart=dcart,
+                        **res, **rules)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -1,0 +1,108 @@
+"""Grid construction and plane-flux arithmetic against the reference's
+integration fixtures (test/c4h4/integration, test/open-shell/integration),
+with the tensors from the CPU oracle: pins the bond-grid geometry, the
+quadrature, the field choice and the +/- flux split on the CPU."""
+import re
+
+import numpy as np
+import pytest
+
+from conftest import golden, write_fixture
+from gimic_amd import grid as gridmod, inp
+from oracle import molio, oracle
+
+NUM = r"[-+]?\d+\.\d+"
+
+
+def stdout_geometry(text):
+    geo = {}
+    for key in ("center", "origin", "basv1", "basv2", "basv3", "lenghts", "magnet"):
+        m = re.search(rf"^ *{key} +({NUM}) +({NUM}) +({NUM})", text, re.M)
+        geo[key] = np.array([float(m.group(i)) for i in (1, 2, 3)])
+    return geo
+
+
+def stdout_blocks(text, label):
+    """(net, pos, neg) of every block whose first line contains `label`"""
+    out = []
+    lines = text.splitlines()
+    for i, l in enumerate(lines):
+        if label in l:
+            net = float(l.split(":")[1])
+            pos = float(lines[i + 1].split(":")[1].split("(")[0])
+            neg = float(lines[i + 2].split(":")[1].split("(")[0])
+            out.append((net, pos, neg))
+    return out
+
+
+def fixture_setup(tmp_path, name, uhf):
+    d = golden(name)
+    mol, xd = write_fixture(str(tmp_path), name)
+    b = molio.read_mol(mol)
+    dens = molio.read_xdens(xd, b, uhf=uhf)
+    kw = inp.with_defaults(inp.parse(str(d["integration_inp"])))
+    g = gridmod.new_grid(dict(kw["Grid"]), np.asarray(b.coords))
+    B = gridmod.get_magnet(g, kw.get("magnet_axis", ""), kw.get("magnet"))
+    return d, b, dens, kw, g, B
+
+
+def flux_sums(J, g, B):
+    """integral.f90:119-143 and 258-283 on precomputed tensors"""
+    jv = np.einsum("pbm,b->pm", J.reshape(-1, 3, 3), B)
+    n = g.normal()
+    dot = jv @ n
+    w = g.weights()
+    r = np.linalg.norm(g.points() - g.center(), axis=1)
+    w = np.where(r > g.radius, 0.0, w)
+    c = dot * w
+    sgn = np.where(np.abs(dot) < 1e-12, 0.0, np.sign(dot))
+    cm = sgn * np.linalg.norm(jv, axis=1) * w
+    split = lambda v: (v.sum(), v[v > 0].sum(), v[v <= 0].sum())
+    return split(c), split(cm)
+
+
+@pytest.mark.parametrize("name,uhf", [("c4h4.npz", False), ("openshell.npz", True)])
+def test_bond_grid_geometry_matches_reference_printout(tmp_path, name, uhf):
+    d, b, dens, kw, g, B = fixture_setup(tmp_path, name, uhf)
+    geo = stdout_geometry(str(d["integration_stdout"]))
+    tol = 5.01e-7  # f12.6
+    assert np.allclose(g.bond_center, geo["center"], atol=tol)
+    assert np.allclose(g.l, geo["lenghts"], atol=tol)
+    for k in range(3):
+        assert np.allclose(g.basv[:, k], geo[f"basv{k + 1}"], atol=tol)
+    assert np.allclose(B, geo["magnet"], atol=tol)
+    # printed origin is before the (zero) rotation and the quadrature
+    m = re.search(r"Adjusted number of grid points for quadrature: +(\d+) +(\d+) +(\d+)",
+                  str(d["integration_stdout"]))
+    assert [int(m.group(1)), int(m.group(2))] == list(g.npts[:2])
+
+
+@pytest.mark.parametrize("name,uhf", [("c4h4.npz", False), ("openshell.npz", True)])
+def test_oracle_flux_matches_reference_integration_fixture(tmp_path, name, uhf):
+    d, b, dens, kw, g, B = fixture_setup(tmp_path, name, uhf)
+    text = str(d["integration_stdout"])
+    cur = stdout_blocks(text, "Induced current (au)")
+    mod = stdout_blocks(text, "Induced mod current (au)")
+    spins = ["total", "alpha", "beta", "spindens"] if uhf else ["total"]
+    pts = g.points()
+    thrs = kw["Advanced"]["screening_thrs"] if kw["Advanced"]["screening"] else None
+    for k, spin in enumerate(spins):
+        J = oracle.jtensor(b.arrays(), dens, pts, spin=spin, screening=thrs)
+        c, m = flux_sums(J, g, B)
+        assert np.allclose(c, cur[k], atol=6e-7), (spin, c, cur[k])  # f13.6
+        if mod:
+            assert np.allclose(m, mod[k], atol=6e-7), (spin, m, mod[k])
+
+
+def test_au2si_conversion_factor():
+    from gimic_amd import flux
+    assert f"{flux.au2si(1.0):13.6f}".strip() == "28.179409"
+
+
+def test_inp_parser_reads_fixture_keywords():
+    d = golden("c4h4.npz")
+    kw = inp.parse(str(d["integration_inp"]))
+    assert kw["calc"] == "integral" and kw["magnet_axis"] == "z"
+    assert kw["Grid"]["_arg"] == "bond" and kw["Grid"]["gauss_order"] == 9
+    assert kw["Grid"]["bond"] == [2, 1] and kw["Grid"]["width"] == [-1.25614, 6.0]
+    assert kw["Advanced"]["screening_thrs"] == 1e-8 and kw["Advanced"]["spherical"] is False

@@ -1,0 +1,183 @@
+"""GPU parity: the HIP library (through its C ABI) against the oracle, the
+flang-reference golden vectors and the reference's own fixtures.
+
+Tolerances: FP64 path <= 1e-10 relative per point (error floor 1e-3 of the
+largest tensor in the set); the mixed split-BF16 path <= 2e-4 per point
+with the same floor (its contraction operands carry ~16 significant bits).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden, rel_err, write_fixture
+from oracle import molio, oracle
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+MIXED_TOL = 2e-4
+
+
+@pytest.fixture(scope="module")
+def ring12():
+    from gimic_amd import synth
+    s = synth.ring12()
+    ctx = s.context(screening=1e-8)
+    yield s, ctx
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def c4h4(c4h4_files):
+    from gimic_amd.lib import Context
+    mol, xd = c4h4_files
+    ctx = Context.from_files(mol, xd, uhf=False, screening=1e-8)
+    b = molio.read_mol(mol)
+    dens = molio.read_xdens(xd, b)
+    yield ctx, b, dens
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def openshell(openshell_files):
+    from gimic_amd.lib import Context
+    mol, xd = openshell_files
+    ctx = Context.from_files(mol, xd, uhf=True, screening=1e-8)
+    b = molio.read_mol(mol)
+    dens = molio.read_xdens(xd, b, uhf=True)
+    yield ctx, b, dens
+    ctx.close()
+
+
+def test_ring12_fp64_matches_flang_reference_all_components(ring12):
+    """spherical basis on the GPU vs the reference fed D_cart = T^T D T"""
+    s, ctx = ring12
+    g = golden("ring12_ref.npz")
+    print("case ring12 fp64 golden", flush=True)
+    for flags in [(1, 1, 1), (0, 1, 1), (1, 0, 1), (1, 1, 0)]:
+        ctx.set_flags(*flags)
+        J = ctx.jtensor(g["points"], precision="fp64")
+        e = rel_err(J, g["jt_%d%d%d" % flags])
+        assert e.max() <= FP64_TOL, (flags, e.max())
+    ctx.set_flags(True, True, True)
+
+
+def test_ring12_unscreened_matches_reference():
+    from gimic_amd import synth
+    s = synth.ring12()
+    g = golden("ring12_ref.npz")
+    with s.context(screening=None) as ctx:
+        J = ctx.jtensor(g["points"])
+    assert rel_err(J, g["jt_noscreen"]).max() <= FP64_TOL
+
+
+def test_c4h4_turbomole_fp64_matches_oracle(c4h4):
+    ctx, b, dens = c4h4
+    pts = np.random.default_rng(1).uniform(-7, 7, (300, 3))
+    print("case c4h4 fp64 oracle", flush=True)
+    J = ctx.jtensor(pts)
+    Jo = oracle.jtensor(b.arrays(), dens, pts)
+    assert rel_err(J, Jo).max() <= FP64_TOL
+
+
+def test_c4h4_readgrid_fixture(c4h4):
+    """test/c4h4/read-grid reference jvec.vtu (B = (0,0,-1), printed 3e20.10)"""
+    ctx, _, _ = c4h4
+    d = golden("c4h4.npz")
+    print("case c4h4 read-grid fixture", flush=True)
+    J = ctx.jtensor(d["readgrid_points"])
+    jv = -J[:, 6:9]
+    ref = d["readgrid_jvec"]
+    assert np.all(np.abs(jv - ref) <= 5e-9 * np.abs(ref) + 1e-10 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("spin", ["total", "alpha", "beta", "spindens"])
+def test_openshell_spin_cases_match_oracle(openshell, spin):
+    ctx, b, dens = openshell
+    pts = np.random.default_rng(4).uniform(-6, 6, (200, 3))
+    print(f"case open-shell {spin}", flush=True)
+    J = ctx.jtensor(pts, spin=spin)
+    Jo = oracle.jtensor(b.arrays(), dens, pts, spin=spin)
+    assert rel_err(J, Jo).max() <= FP64_TOL
+
+
+def test_openshell_3d_fixture(openshell):
+    """test/open-shell/3d: 33^3 even grid on [-8,8]^3, B = +z, jvec*.vti (e14.6)"""
+    ctx, _, _ = openshell
+    d = golden("openshell.npz")
+    n = 33
+    ax = -8.0 + 0.5 * np.arange(n)
+    z, y, x = np.meshgrid(ax, ax, ax, indexing="ij")
+    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    for spin, key in [("total", "jvec"), ("alpha", "jvecalpha"), ("beta", "jvecbeta"),
+                      ("spindens", "jvecspindens")]:
+        print(f"case open-shell 3d {spin}", flush=True)
+        J = ctx.jtensor(pts, spin=spin)
+        jv = J[:, 6:9].ravel()
+        ref = d[key].astype(np.float64)
+        # e14.6 keeps six significant digits: rounding error up to 5e-6 relative
+        ok = np.abs(jv - ref) <= 5.1e-6 * np.abs(ref) + 1e-16
+        assert ok.mean() == 1.0, (spin, np.abs(jv - ref).max())
+
+
+@pytest.mark.parametrize("npts", [0, 1, 127, 129, 1000])
+def test_ragged_batch_sizes(ring12, npts):
+    s, ctx = ring12
+    g = golden("ring12_ref.npz")
+    pts = np.resize(g["points"], (npts, 3)) if npts else np.zeros((0, 3))
+    J = ctx.jtensor(pts)
+    assert J.shape == (npts, 9)
+    if npts:
+        ref = np.resize(g["jt_111"], (npts, 9))
+        assert rel_err(J, ref).max() <= FP64_TOL
+
+
+def test_mixed_precision_within_stated_tolerance(ring12, c4h4):
+    s, ctx = ring12
+    g = golden("ring12_ref.npz")
+    print("case mixed ring12", flush=True)
+    Jm = ctx.jtensor(g["points"], precision="bf16x3")
+    e = rel_err(Jm, g["jt_111"])
+    print("ring12 mixed per-point error max %.3e p99 %.3e" % (e.max(), np.quantile(e, 0.99)))
+    assert e.max() <= MIXED_TOL
+    ctx4, b, dens = c4h4
+    pts = np.random.default_rng(9).uniform(-7, 7, (500, 3))
+    Jm = ctx4.jtensor(pts, precision="bf16x3")
+    J = ctx4.jtensor(pts, precision="fp64")
+    e = rel_err(Jm, J)
+    print("c4h4 mixed per-point error max %.3e p99 %.3e" % (e.max(), np.quantile(e, 0.99)))
+    assert e.max() <= MIXED_TOL
+
+
+def test_beta_on_closed_shell_is_an_error(ring12):
+    from gimic_amd.lib import GimicError
+    s, ctx = ring12
+    with pytest.raises(GimicError, match="not open-shell"):
+        ctx.jtensor(np.zeros((4, 3)), spin="beta")
+
+
+def test_drop_in_c_abi(c4h4_files):
+    """gimic_init / set_magnet / set_spin / calc_jtensor / calc_jvector as the
+    reference's C interface (gimic_interface.f90:26-151)"""
+    from gimic_amd import lib
+    L = lib.load()
+    mol, xd = c4h4_files
+    L.gimic_init(mol.encode(), xd.encode())
+    b = molio.read_mol(mol)
+    dens = molio.read_xdens(xd, b)
+    pts = np.random.default_rng(5).uniform(-5, 5, (8, 3))
+    Jo = oracle.jtensor(b.arrays(), dens, pts, screening=1e-6)  # C-ABI threshold
+    B = np.array([0.3, -0.5, 0.8])
+    L.gimic_set_magnet(lib.ptr(B))
+    L.gimic_set_spin(b"total")
+    for k in range(len(pts)):
+        r = np.ascontiguousarray(pts[k])
+        jt = np.zeros(9)
+        jv = np.zeros(3)
+        L.gimic_calc_jtensor(lib.ptr(r), lib.ptr(jt))
+        L.gimic_calc_jvector(lib.ptr(r), lib.ptr(jv))
+        assert rel_err(jt[None], Jo[k:k + 1], floor_frac=1.0).max() <= FP64_TOL
+        assert np.allclose(jv, Jo[k].reshape(3, 3, order="F") @ B, rtol=1e-10, atol=1e-16)
+    L.gimic_finalize()
