@@ -81,19 +81,6 @@ struct DevBuf {
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
-uint16_t host_bf16(float x) {
-    uint32_t u;
-    std::memcpy(&u, &x, 4);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-float host_bf16_f(uint16_t h) {
-    uint32_t u = ((uint32_t)h) << 16;
-    float f;
-    std::memcpy(&f, &u, 4);
-    return f;
-}
-
 }  // namespace
 
 struct gimic_ctx {
@@ -104,20 +91,24 @@ struct gimic_ctx {
     int giao = 1, diamag = 1, paramag = 1;
     int nset = 0;
     std::vector<double> dens;  // nset*4*n*n column-major, as given
-    int lda = 0, ncol_pad = 0, ntiles = 0;
+    int lda = 0, ncol_pad = 0;
     DevBuf shells, exps, ncc, coords, rf64, rf32;
     // right operands per (spin list, precision)
     struct Mats {
-        DevBuf b64, bhi, blo;
+        DevBuf b64, b32;
         int nmatset = 0;
     };
     std::map<std::pair<std::vector<int>, int>, std::unique_ptr<Mats>> mats;
     // workspace
-    DevBuf phi64, grad64, ahi, alo, phi32, grad32, part, pts_tmp, jt_tmp, contrib, planes,
+    DevBuf phi64, grad64, phi32, grad32, part, pts_tmp, jt_tmp, contrib, planes,
         pparams, offs, outs;
     // optional per-kernel timing with HIP events on the launch stream:
     // [0] basis evaluation, [1] contraction, [2] finalize
     bool timing = false;
+    // mixed-precision contraction kernel chosen by timing (-1: not yet tuned;
+    // 1 register-tiled, 2 LDS-tiled) and the timings that chose it
+    int mixed_variant = -1;
+    float tune_ms[2] = {0.f, 0.f};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
     double ms[3] = {0.0, 0.0, 0.0};
     long long launches[3] = {0, 0, 0};
@@ -143,8 +134,7 @@ void setup_device(gimic_ctx *c) {
     Basis &b = c->basis;
     finalize_basis(b, c->screen_thrs);
     c->lda = (int)round_up((size_t)b.nbf, kKPad);
-    c->ncol_pad = (int)round_up((size_t)b.nbf, kNT);
-    c->ntiles = c->ncol_pad / kNT;
+    c->ncol_pad = (int)round_up((size_t)b.nbf, 32);
     std::vector<DevShell> sh;
     for (auto &s : b.shells)
         sh.push_back(DevShell{s.atom, s.l, s.nprim, s.poff, s.fout,
@@ -220,17 +210,22 @@ gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec)
     for (size_t si = 0; si < spins.size(); si++) {
         std::vector<double> fam = spin_family(c, spins[si]);
         double *Bs = &B[si * setsz];
-        // matrix j column nu (K = mu contiguous): D(mu,nu) = fam[blk + mu + n*nu]
-        for (size_t nu = 0; nu < n; nu++)
+        // matrix j column nu (K = mu contiguous): D(mu,nu) = fam[blk + mu + n*nu].
+        // The GIAO right operands are W_d(mu,nu) = D(mu,nu) (R_nu - R_mu)_d:
+        // phi W_d equals R_d (phi D) - phi diag(R_d) D without the
+        // cancellation between those two terms (and is zero on one-atom blocks).
+        for (size_t nu = 0; nu < n; nu++) {
+            const double *Rn = &c->basis.coords[3 * atom_of[nu]];
             for (size_t mu = 0; mu < n; mu++) {
                 const double D = fam[mu + n * nu];
+                const double *Rm = &c->basis.coords[3 * atom_of[mu]];
                 Bs[0 * matsz + nu * ld + mu] = D;
                 for (int b = 1; b < 4; b++)
                     Bs[b * matsz + nu * ld + mu] = fam[(size_t)b * n * n + mu + n * nu];
                 for (int d = 0; d < 3; d++)
-                    Bs[(4 + d) * matsz + nu * ld + mu] =
-                        c->basis.coords[3 * atom_of[mu] + d] * D;
+                    Bs[(4 + d) * matsz + nu * ld + mu] = D * (Rn[d] - Rm[d]);
             }
+        }
     }
     hipStream_t st = c->stream;
     if (prec == GIMIC_PREC_FP64) {
@@ -239,17 +234,10 @@ gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec)
                                  hipMemcpyHostToDevice, st),
                   "hipMemcpyAsync");
     } else {
-        std::vector<uint16_t> hi(B.size()), lo(B.size());
-        for (size_t i = 0; i < B.size(); i++) {
-            const float x = (float)B[i];
-            hi[i] = host_bf16(x);
-            lo[i] = host_bf16(x - host_bf16_f(hi[i]));
-        }
-        m->bhi.alloc(hi.size() * 2, st, false);
-        m->blo.alloc(lo.size() * 2, st, false);
-        hip_check(hipMemcpyAsync(m->bhi.p, hi.data(), hi.size() * 2, hipMemcpyHostToDevice, st),
-                  "hipMemcpyAsync");
-        hip_check(hipMemcpyAsync(m->blo.p, lo.data(), lo.size() * 2, hipMemcpyHostToDevice, st),
+        std::vector<float> b32(B.begin(), B.end());
+        m->b32.alloc(b32.size() * sizeof(float), st, false);
+        hip_check(hipMemcpyAsync(m->b32.p, b32.data(), b32.size() * sizeof(float),
+                                 hipMemcpyHostToDevice, st),
                   "hipMemcpyAsync");
     }
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -286,7 +274,7 @@ struct FluxDev {
 void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
               const std::vector<int> &spins, int prec, hipStream_t st, const FluxDev *flux) {
     if (npts <= 0) return;
-    if (prec != GIMIC_PREC_FP64 && prec != GIMIC_PREC_BF16X3)
+    if (prec != GIMIC_PREC_FP64 && prec != GIMIC_PREC_MIXED)
         throw Error(GIMIC_EINVAL, "unknown precision");
     for (int s : spins)
         if (s < 0 || s > 3) throw Error(GIMIC_EINVAL, "invalid spin case");
@@ -294,32 +282,25 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
     const size_t ld = (size_t)c->lda;
     // chunk of points per pass: basis values + gradients of one chunk stay
     // within ~4 GiB (FP64: 32 bytes per point and function)
-    const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 20);
+    const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 16);
+    // row granularity: 128 for the FP64 kernel, 256 for the mixed kernels
+    const size_t gran = prec == GIMIC_PREC_FP64 ? (size_t)kWgRows : (size_t)contract_mx_rows();
     size_t cap = (size_t)4 << 30;
-    size_t chunk = std::max<size_t>(kWgRows, (cap / per_row) / kWgRows * kWgRows);
-    chunk = std::min(chunk, round_up((size_t)npts, kWgRows));
+    size_t chunk = std::max<size_t>(gran, (cap / per_row) / gran * gran);
+    chunk = std::min(chunk, round_up((size_t)npts, gran));
     const size_t rows_cap = chunk;
     const long long gplane = (long long)(rows_cap * ld);
     if (prec == GIMIC_PREC_FP64) {
         c->phi64.alloc(rows_cap * ld * 8, st, c->phi64.bytes < rows_cap * ld * 8);
         c->grad64.alloc(3 * rows_cap * ld * 8, st, c->grad64.bytes < 3 * rows_cap * ld * 8);
     } else {
-        c->ahi.alloc(rows_cap * ld * 2, st, c->ahi.bytes < rows_cap * ld * 2);
-        c->alo.alloc(rows_cap * ld * 2, st, c->alo.bytes < rows_cap * ld * 2);
         c->phi32.alloc(rows_cap * ld * 4, st, c->phi32.bytes < rows_cap * ld * 4);
         c->grad32.alloc(3 * rows_cap * ld * 4, st, c->grad32.bytes < 3 * rows_cap * ld * 4);
     }
     const int nmatset = M.nmatset;
     for (size_t start = 0; start < (size_t)npts; start += chunk) {
         const int n_c = (int)std::min(chunk, (size_t)npts - start);
-        const int rows = (int)round_up((size_t)n_c, kWgRows);
-        const int wg_rows = rows / kWgRows;
-        int nsplit = (int)std::min<long long>(
-            c->ntiles, std::max<long long>(1, (2048 + (long long)wg_rows * nmatset - 1) /
-                                                  ((long long)wg_rows * nmatset)));
-        const int tps = (c->ntiles + nsplit - 1) / nsplit;
-        nsplit = (c->ntiles + tps - 1) / tps;
-        c->part.alloc((size_t)nmatset * nsplit * kNQ * rows * 8, st, false);
+        const int rows = (int)round_up((size_t)n_c, gran);
 
         EvalArgs ea{};
         ea.pts = d_r + 3 * start;
@@ -336,8 +317,6 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         ea.lda = c->lda;
         ea.phi64 = c->phi64.as<double>();
         ea.grad64 = c->grad64.as<double>();
-        ea.ahi = c->ahi.as<uint16_t>();
-        ea.alo = c->alo.as<uint16_t>();
         ea.phi32 = c->phi32.as<float>();
         ea.grad32 = c->grad32.as<float>();
         timed(c, 0, st, [&] {
@@ -349,28 +328,68 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         ca.lda = c->lda;
         ca.ncol_pad = c->ncol_pad;
         ca.gplane = gplane;
-        ca.ntiles = c->ntiles;
-        ca.nsplit = nsplit;
-        ca.tiles_per_split = tps;
         ca.nmatset = nmatset;
+        ca.pts = d_r + 3 * start;
+        ca.npts = n_c;
         ca.a64 = c->phi64.as<double>();
         ca.g64 = c->grad64.as<double>();
         ca.b64 = M.b64.as<double>();
-        ca.ahi = c->ahi.as<uint16_t>();
-        ca.alo = c->alo.as<uint16_t>();
-        ca.bhi = M.bhi.as<uint16_t>();
-        ca.blo = M.blo.as<uint16_t>();
-        ca.phi32 = c->phi32.as<float>();
+        ca.a32 = c->phi32.as<float>();
         ca.g32 = c->grad32.as<float>();
+        ca.b32 = M.b32.as<float>();
         ca.rf64 = c->rf64.as<double>();
         ca.rf32 = c->rf32.as<float>();
-        ca.part = c->part.as<double>();
-        timed(c, 1, st, [&] {
-            if (prec == GIMIC_PREC_FP64)
+        // work split over N so a launch has at least ~2048 workgroups
+        auto plan = [&](int wg_rows, int tile_cols) {
+            ca.ntiles = c->ncol_pad / tile_cols;
+            const long long per = (long long)(rows / wg_rows) * nmatset;
+            int nsplit = (int)std::min<long long>(ca.ntiles,
+                                                  std::max<long long>(1, (2048 + per - 1) / per));
+            ca.tiles_per_split = (ca.ntiles + nsplit - 1) / nsplit;
+            ca.nsplit = (ca.ntiles + ca.tiles_per_split - 1) / ca.tiles_per_split;
+            c->part.alloc((size_t)nmatset * ca.nsplit * kNQ * rows * 8, st, false);
+            ca.part = c->part.as<double>();
+        };
+        auto run = [&](int variant) {
+            if (variant == 0) {
+                plan(kWgRows, kNT);
                 hip_check(launch_contract_f64(ca, st), "contract_f64");
-            else
-                hip_check(launch_contract_bf16x3(ca, st), "contract_bf16x3");
-        });
+            } else if (variant == 1) {
+                plan(kWgRows, kNT);
+                hip_check(launch_contract_bf16x6(ca, st), "contract_bf16x6");
+            } else {
+                plan(contract_mx_rows(), contract_mx_cols());
+                hip_check(launch_contract_mx(ca, st), "contract_mx");
+            }
+        };
+        if (prec == GIMIC_PREC_FP64) {
+            timed(c, 1, st, [&] { run(0); });
+        } else {
+            if (c->mixed_variant < 0) {
+                // autotune once per context: time both mixed kernels on this chunk
+                float best = 0.f;
+                for (int v = 1; v <= 2; v++) {
+                    hipEvent_t e0, e1;
+                    hip_check(hipEventCreate(&e0), "hipEventCreate");
+                    hip_check(hipEventCreate(&e1), "hipEventCreate");
+                    hip_check(hipEventRecord(e0, st), "hipEventRecord");
+                    run(v);
+                    hip_check(hipEventRecord(e1, st), "hipEventRecord");
+                    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                    float ms = 0.f;
+                    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    c->tune_ms[v - 1] = ms;
+                    if (c->mixed_variant < 0 || ms < best) {
+                        best = ms;
+                        c->mixed_variant = v;
+                    }
+                }
+            }
+            timed(c, 1, st, [&] { run(c->mixed_variant); });
+        }
+        const int nsplit = ca.nsplit;
 
         FinalizeArgs fa{};
         fa.part = c->part.as<double>();
@@ -524,6 +543,15 @@ int gimic_ctx_get_timing(gimic_ctx *c, double *ms, long long *launches) {
             ms[k] = c->ms[k];
             launches[k] = c->launches[k];
         }
+    });
+}
+
+int gimic_ctx_tuning(const gimic_ctx *c, int *variant, double *ms) {
+    return guarded([&] {
+        if (!c || !variant || !ms) throw Error(GIMIC_EINVAL, "null argument");
+        *variant = c->mixed_variant;
+        ms[0] = c->tune_ms[0];
+        ms[1] = c->tune_ms[1];
     });
 }
 

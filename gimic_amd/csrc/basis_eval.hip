@@ -10,8 +10,8 @@
 //   d_m phi = k_m r_m^(k_m-1)(...) g - 2 r_m x^a y^b z^c h (caos.f90:39-64)
 // with contractions outside their screening radius set to zero
 // (basis.f90:118-136).  Spherical contractions are T_l times the cartesian
-// ones.  Outputs are written as the contraction's left operand (FP64, or a
-// BF16 hi/lo split) plus phi and the gradient for the epilogue.
+// ones.  Outputs: phi (the contraction's left operand) and its gradient, in
+// FP64 for the FP64 path or FP32 for the mixed path.
 #include "device_common.h"
 
 namespace gimic {
@@ -77,12 +77,7 @@ __device__ __forceinline__ void store(const EvalArgs &a, size_t row, int f, doub
         a.grad64[plane + i] = gy;
         a.grad64[2 * plane + i] = gz;
     } else {
-        const float v32 = (float)v;
-        const uint16_t hi = f32_to_bf16_rn(v32);
-        const uint16_t lo = f32_to_bf16_rn(v32 - bf16_to_f32(hi));
-        a.ahi[i] = hi;
-        a.alo[i] = lo;
-        a.phi32[i] = v32;
+        a.phi32[i] = (float)v;
         a.grad32[i] = (float)gx;
         a.grad32[plane + i] = (float)gy;
         a.grad32[2 * plane + i] = (float)gz;

@@ -5,10 +5,15 @@
 // length-n GEMVs phi*D, phi*DB_b, db_b*D and 27 dot products.  db_b = S_b phi
 // with S = r x R_A (bfeval.f90:168-293), so every GEMV shares the left
 // operand phi and a tile of P points is one GEMM
-//     Y[P x 7n] = Phi[P x n] * [D | DBx | DBy | DBz | RxD | RyD | RzD]
+//     Y[P x 7n] = Phi[P x n] * [D | DBx | DBy | DBz | Wx | Wy | Wz]
 // followed by reductions over the output column nu against phi_nu,
 // grad phi_nu and R_nu (the kNQ partial sums listed in device_common.h).
 // The finalize kernel turns the partials into the 3x3 tensor.
+//
+// With W_d(mu,nu) = D(mu,nu)(R_nu - R_mu)_d in place of R_d D, the GIAO
+// reduction needs no difference of large terms, and the diamagnetic term
+// is accumulated as sum Y0 phi (r - R_nu) (gauge origin at each function's
+// centre), again free of cancellation.
 //
 // Tiling: 256 threads = 4 wavefronts; a workgroup owns 128 point rows
 // (32 per wavefront: two 16-row MFMA blocks) and walks a range of 16-column
@@ -81,25 +86,25 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs a) {
             for (int r = 0; r < 4; r++) {
                 const int rl = wave * 32 + rb * 16 + g + 4 * r;
                 const size_t p = row0 + rl;
+                const bool valid = p < (size_t)a.npts;
+                const double rp[3] = {valid ? a.pts[3 * p] : 0.0, valid ? a.pts[3 * p + 1] : 0.0,
+                                      valid ? a.pts[3 * p + 2] : 0.0};
                 const double ph = a.a64[p * lda + col];
                 const double G[3] = {a.g64[p * lda + col], a.g64[gplane + p * lda + col],
                                      a.g64[2 * gplane + p * lda + col]};
                 const double y0 = acc[rb][0][r];
                 double q[kNQ];
                 q[0] = y0 * ph;
-                q[1] = q[0] * R[0];
-                q[2] = q[0] * R[1];
-                q[3] = q[0] * R[2];
+#pragma unroll
+                for (int d = 0; d < 3; d++) q[1 + d] = q[0] * (rp[d] - R[d]);
 #pragma unroll
                 for (int bb = 0; bb < 3; bb++)
 #pragma unroll
                     for (int m = 0; m < 3; m++) q[4 + m + 3 * bb] = acc[rb][1 + bb][r] * G[m];
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const double t = R[d] * y0 - acc[rb][4 + d][r];
+                for (int d = 0; d < 3; d++)
 #pragma unroll
-                    for (int m = 0; m < 3; m++) q[13 + m + 3 * d] = G[m] * t;
-                }
+                    for (int m = 0; m < 3; m++) q[13 + m + 3 * d] = G[m] * acc[rb][4 + d][r];
 #pragma unroll
                 for (int off = 1; off < 16; off <<= 1)
 #pragma unroll

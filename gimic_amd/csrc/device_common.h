@@ -12,14 +12,15 @@ typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
 // Number of per-point partial sums produced by the contraction epilogue:
 //   0        rho   = sum_nu Y0 phi
-//   1..3     C_d   = sum_nu Y0 phi R_d
-//   4..12    A_mb  = sum_nu Yb_b G_m            (index 4 + m + 3b)
-//   13..21   Q_md  = sum_nu G_m (R_d Y0 - Z_d)  (index 13 + m + 3d)
-// with Y0 = Phi D, Yb_b = Phi DB_b, Z_d = Phi diag(R_d) D.
+//   1..3     E_c   = sum_nu Y0 phi (r - R_nu)_c   (= rho r_c - C_c)
+//   4..12    A_mb  = sum_nu Yb_b G_m              (index 4 + m + 3b)
+//   13..21   Q_md  = sum_nu G_m Yw_d              (index 13 + m + 3d)
+// with Y0 = Phi D, Yb_b = Phi DB_b, Yw_d = Phi W_d, W_d(mu,nu) =
+// D(mu,nu) (R_nu - R_mu)_d, R_nu the centre of function nu and r the point.
 constexpr int kNQ = 22;
 
 // Matrices in the contraction's right operand, per density set.
-constexpr int kNMat = 7;  // D, DBx, DBy, DBz, RxD, RyD, RzD
+constexpr int kNMat = 7;  // D, DBx, DBy, DBz, Wx, Wy, Wz
 
 // Point rows per workgroup tile and columns per N-tile of the contraction.
 constexpr int kWgRows = 128;
@@ -62,8 +63,7 @@ struct EvalArgs {
     int lda;             // padded K (multiple of kKPad)
     // FP64 outputs
     double *phi64, *grad64;  // [rows][lda], [3][rows][lda]
-    // mixed outputs
-    uint16_t *ahi, *alo;     // [rows][lda] bf16 split of phi
+    // mixed outputs (FP32 values; the contraction splits them into BF16)
     float *phi32, *grad32;   // [rows][lda], [3][rows][lda]
 };
 hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s);
@@ -74,15 +74,19 @@ struct ContractArgs {
     long long gplane;    // elements between the three gradient planes
     int ntiles, nsplit, tiles_per_split;
     int nmatset;      // density sets packed side by side (7 matrices each)
-    const double *a64, *g64, *b64;          // FP64 path
-    const uint16_t *ahi, *alo, *bhi, *blo;  // split-BF16 path
-    const float *phi32, *g32;
+    const double *pts;   // chunk points (3 per row, npts valid rows)
+    int npts;
+    const double *a64, *g64, *b64;  // FP64 path
+    const float *a32, *g32, *b32;   // mixed path (FP32 operands, split to BF16 x3)
     const double *rf64;  // [3][ncol_pad] atom coordinate of each function
     const float *rf32;
     double *part;        // [nmatset][nsplit][kNQ][rows]
 };
 hipError_t launch_contract_f64(const ContractArgs &a, hipStream_t s);
-hipError_t launch_contract_bf16x3(const ContractArgs &a, hipStream_t s);
+hipError_t launch_contract_bf16x6(const ContractArgs &a, hipStream_t s);
+hipError_t launch_contract_mx(const ContractArgs &a, hipStream_t s);
+int contract_mx_rows();  // row tile of launch_contract_mx (a multiple of kWgRows)
+int contract_mx_cols();  // functions per N-tile of launch_contract_mx
 
 struct FinalizeArgs {
     const double *part;  // [nmatset][nsplit][kNQ][rows]

@@ -34,33 +34,50 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
 #pragma unroll
             for (int i = 0; i < kNQ; i++) q[i] += src[(size_t)i * a.rows];
         }
+        // q[1+c] = E_c = rho r_c - C_c, C_c = sum_nu Y0 phi R_nu,c.  With all
+        // three terms on, eps_bmd C_d + eps_mbc rho r_c = eps_mbd E_d, so the
+        // tensor is assembled without forming rho r and C separately.
         const double rho = q[0];
+        const bool full = a.giao && a.paramag && a.diamag;
         double J[9];
 #pragma unroll
         for (int b = 0; b < 3; b++)
 #pragma unroll
             for (int m = 0; m < 3; m++) {
-                double giao = 0.0;
+                double rq = 0.0;  // eps_bcd r_c Q_md
 #pragma unroll
                 for (int cc = 0; cc < 3; cc++)
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
                         const double e = eps3(b, cc, d);
-                        if (e != 0.0) giao += e * r[cc] * q[13 + m + 3 * d];
+                        if (e != 0.0) rq += e * r[cc] * q[13 + m + 3 * d];
                     }
+                double v;
+                if (full) {
+                    double ee = 0.0;
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const double e = eps3(b, m, d);
-                    if (e != 0.0) giao += e * q[1 + d];
-                }
-                double para = q[4 + m + 3 * b];
-                if (a.giao) para += giao;
-                double v = a.paramag ? 0.5 * para : 0.0;
-                if (a.diamag) {
+                    for (int d = 0; d < 3; d++) {
+                        const double e = eps3(m, b, d);
+                        if (e != 0.0) ee += e * q[1 + d];
+                    }
+                    v = 0.5 * (q[4 + m + 3 * b] + rq) + 0.5 * ee;
+                } else {
+                    double para = q[4 + m + 3 * b];
+                    if (a.giao) {
+                        para += rq;
 #pragma unroll
-                    for (int cc = 0; cc < 3; cc++) {
-                        const double e = eps3(m, b, cc);
-                        if (e != 0.0) v += 0.5 * e * rho * r[cc];
+                        for (int d = 0; d < 3; d++) {
+                            const double e = eps3(b, m, d);
+                            if (e != 0.0) para += e * (rho * r[d] - q[1 + d]);
+                        }
+                    }
+                    v = a.paramag ? 0.5 * para : 0.0;
+                    if (a.diamag) {
+#pragma unroll
+                        for (int cc = 0; cc < 3; cc++) {
+                            const double e = eps3(m, b, cc);
+                            if (e != 0.0) v += 0.5 * e * rho * r[cc];
+                        }
                     }
                 }
                 J[m + 3 * b] = v;

@@ -10,7 +10,7 @@ and runs
 printing the reference's stdout blocks.  Both integrals come from one tensor
 evaluation per spin case.
 
-    python -m gimic_amd.driver gimic.inp [--precision fp64|bf16x3]
+    python -m gimic_amd.driver gimic.inp [--precision fp64|mixed]
 """
 import argparse
 import os
@@ -81,7 +81,7 @@ def run_cdens(kw, ctx, g, b, uhf, precision="fp64"):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("input", nargs="?", default="gimic.inp")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "bf16x3"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     a = ap.parse_args(argv)
     kw = inp.parse(open(a.input).read())
     kw, ctx, g, b, uhf = setup(kw, os.path.dirname(os.path.abspath(a.input)))

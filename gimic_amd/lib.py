@@ -12,14 +12,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libgimic_hip.so")
 
 SPIN = {"total": 0, "alpha": 1, "beta": 2, "spindens": 3}
-PREC = {"fp64": 0, "bf16x3": 1, "mixed": 1}
+PREC = {"fp64": 0, "mixed": 1, "bf16x6": 1}
 
 EXPORTS = [
     "gimic_init", "gimic_finalize", "gimic_set_uhf", "gimic_set_magnet", "gimic_set_spin",
     "gimic_set_screening", "gimic_calc_jtensor", "gimic_calc_jvector", "gimic_calc_modj",
     "gimic_get_gauss_points", "mkgausspoints", "gimic_last_error", "gimic_ctx_create_files",
     "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
-    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
+    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
     "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_sph_transform", "gimic_quadrature",
     "gimic_selftest_mfma_layout",
 ]
@@ -62,6 +62,7 @@ def load():
         "gimic_ctx_atoms": [P, P, P],
         "gimic_ctx_set_timing": [P, I],
         "gimic_ctx_get_timing": [P, P, P],
+        "gimic_ctx_tuning": [P, P, P],
         "gimic_jtensor_batch": [P, I64, P, P, I, I],
         "gimic_jtensor_batch_device": [P, I64, P, P, I, I, P],
         "gimic_jtensor_multi_device": [P, I64, P, P, I, P, I, P],
@@ -186,6 +187,13 @@ class Context:
         check(self._lib.gimic_ctx_get_timing(self._h, ptr(ms), ptr(n)))
         return {k: (float(ms[i]), int(n[i]))
                 for i, k in enumerate(["basis_eval", "contract", "finalize"])}
+
+    def tuning(self):
+        """(chosen mixed kernel variant, [ms variant 1, ms variant 2])"""
+        v = ctypes.c_int()
+        ms = np.zeros(2)
+        check(self._lib.gimic_ctx_tuning(self._h, ctypes.byref(v), ptr(ms)))
+        return v.value, ms.tolist()
 
     def atoms(self):
         """(coords (natoms,3) bohr, charges (natoms,))"""

@@ -74,10 +74,11 @@ enum {
 
 enum { GIMIC_SPIN_TOTAL = 0, GIMIC_SPIN_ALPHA = 1, GIMIC_SPIN_BETA = 2, GIMIC_SPIN_SPINDENS = 3 };
 
-/* contraction precision: FP64 MFMA, or split-BF16 (hi*hi + hi*lo + lo*hi)
- * MFMA with FP32 accumulation; basis values, partial sums across tiles, the
- * tensor assembly and flux sums stay FP64 in both */
-enum { GIMIC_PREC_FP64 = 0, GIMIC_PREC_BF16X3 = 1 };
+/* contraction precision: FP64 MFMA, or mixed: FP32 operands each split into
+ * three BF16 parts with the six products of split order <= 2 on BF16 MFMA,
+ * FP32 accumulation.  Basis-function exponentials, partial sums across
+ * tiles, the tensor assembly and the flux sums are FP64 in both. */
+enum { GIMIC_PREC_FP64 = 0, GIMIC_PREC_MIXED = 1 };
 
 /* message of the last failing call on this thread ("" if none) */
 const char *gimic_last_error(void);
@@ -122,6 +123,12 @@ int gimic_ctx_atoms(const gimic_ctx *ctx, double *coords, double *charges);
  * in that order).  Turning timing on or off resets the totals. */
 int gimic_ctx_set_timing(gimic_ctx *ctx, int on);
 int gimic_ctx_get_timing(gimic_ctx *ctx, double *ms, long long *launches);
+
+/* Mixed-precision kernel selection (no counterpart in the reference): the
+ * first mixed evaluation of a context times the register-tiled (1) and the
+ * LDS-tiled (2) contraction kernels on its first chunk and keeps the faster
+ * one.  Returns the choice (-1 before the first mixed call) and both times. */
+int gimic_ctx_tuning(const gimic_ctx *ctx, int *variant, double *ms);
 
 int gimic_ctx_destroy(gimic_ctx *ctx);
 

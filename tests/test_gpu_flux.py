@@ -77,7 +77,7 @@ def test_mixed_precision_flux_error_reported(tmp_path):
     g = gridmod.new_grid(kw, np.asarray(s.coords))
     B = gridmod.get_magnet(g, "X")
     f64 = np.array(flux.integrate_current(ctx, g, B, precision="fp64"))
-    mix = np.array(flux.integrate_current(ctx, g, B, precision="bf16x3"))
+    mix = np.array(flux.integrate_current(ctx, g, B, precision="mixed"))
     rel = np.abs(mix - f64) / np.abs(f64)
     print("ring12 plane flux fp64", f64, "mixed", mix, "rel err", rel)
     assert rel[0] <= 1e-6

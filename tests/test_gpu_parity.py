@@ -2,8 +2,8 @@
 flang-reference golden vectors and the reference's own fixtures.
 
 Tolerances: FP64 path <= 1e-10 relative per point (error floor 1e-3 of the
-largest tensor in the set); the mixed split-BF16 path <= 2e-4 per point
-with the same floor (its contraction operands carry ~16 significant bits).
+largest tensor in the set); the mixed path (FP32 operands split into three BF16 parts) <= 1e-5 per point
+with the same floor.
 """
 import ctypes
 import os
@@ -17,7 +17,7 @@ from oracle import molio, oracle
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-10
-MIXED_TOL = 2e-4
+MIXED_TOL = 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -138,13 +138,13 @@ def test_mixed_precision_within_stated_tolerance(ring12, c4h4):
     s, ctx = ring12
     g = golden("ring12_ref.npz")
     print("case mixed ring12", flush=True)
-    Jm = ctx.jtensor(g["points"], precision="bf16x3")
+    Jm = ctx.jtensor(g["points"], precision="mixed")
     e = rel_err(Jm, g["jt_111"])
     print("ring12 mixed per-point error max %.3e p99 %.3e" % (e.max(), np.quantile(e, 0.99)))
     assert e.max() <= MIXED_TOL
     ctx4, b, dens = c4h4
     pts = np.random.default_rng(9).uniform(-7, 7, (500, 3))
-    Jm = ctx4.jtensor(pts, precision="bf16x3")
+    Jm = ctx4.jtensor(pts, precision="mixed")
     J = ctx4.jtensor(pts, precision="fp64")
     e = rel_err(Jm, J)
     print("c4h4 mixed per-point error max %.3e p99 %.3e" % (e.max(), np.quantile(e, 0.99)))
