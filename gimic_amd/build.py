@@ -11,8 +11,8 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libgimic_hip.so")
 ARCH = os.environ.get("GIMIC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["basis_eval.hip", "contract_f64.hip", "contract_bf16.hip", "contract_mx.hip", "finalize.hip", "selftest.hip",
-           "api.cpp", "host_basis.cpp", "quadrature.cpp"]
+SOURCES = ["basis_eval.hip", "contract_f64.hip", "contract_bf16.hip", "contract_mx.hip", "finalize.hip", "selftest.hip", "surrogate.hip",
+           "api.cpp", "surrogate_api.cpp", "host_basis.cpp", "quadrature.cpp"]
 
 # Packed FP32 VALU ops stay disabled in device code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -56,5 +56,24 @@ def build(verbose=False):
     return LIB
 
 
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
+FDIR = os.path.join(HERE, "fortran")
+
+
+def build_fortran():
+    """Fortran iso_c_binding module and demo caller (skipped without flang)."""
+    if not os.path.exists(FLANG):
+        return None
+    out = os.path.join(FDIR, "fgimic_hip_demo")
+    cmd = [FLANG, "-O2", "-module-dir", OBJ, os.path.join(FDIR, "gimic_hip.f90"),
+           os.path.join(FDIR, "fgimic_hip_demo.f90"), "-L", HERE, "-lgimic_hip",
+           "-Wl,-rpath,$ORIGIN/..", "-o", out]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("flang failed:\n" + res.stderr)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran())

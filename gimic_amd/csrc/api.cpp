@@ -19,9 +19,17 @@
 
 using namespace gimic;
 
+namespace gimic {
+// message of the last failing C-ABI call on this thread (shared by all entry points)
+std::string &error_slot() {
+    static thread_local std::string s;
+    return s;
+}
+}  // namespace gimic
+
 namespace {
 
-thread_local std::string g_error;
+std::string &g_error = gimic::error_slot();
 
 struct Error : std::runtime_error {
     int code;
@@ -268,6 +276,7 @@ struct FluxDev {
     const double *pnorm = nullptr, *pbfield = nullptr, *pcenter = nullptr, *pradius = nullptr;
     double *contrib = nullptr;
     double *contrib_mod = nullptr;
+    double *contrib_acid = nullptr;
 };
 
 // Evaluate J for npts device points into d_jt (nspin blocks of 9*npts).
@@ -412,6 +421,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             fa.pradius = flux->pradius;
             fa.contrib = flux->contrib + start;
             fa.contrib_mod = flux->contrib_mod ? flux->contrib_mod + start : nullptr;
+            fa.contrib_acid = flux->contrib_acid ? flux->contrib_acid + start : nullptr;
         }
         timed(c, 2, st, [&] { hip_check(launch_finalize(fa, st), "finalize"); });
     }
@@ -596,7 +606,7 @@ int gimic_jtensor_batch(gimic_ctx *c, int64_t npts, const double *r, double *jt,
 int gimic_flux_planes(gimic_ctx *c, int nplanes, const int64_t *offs, const double *r,
                       const double *w, const double *normals, const double *bfields,
                       const double *centers, const double *radii, int spin, int precision,
-                      double *out, double *out_mod, double *jt) {
+                      double *out, double *out_mod, double *out_acid, double *jt) {
     return guarded([&] {
         if (!c || nplanes < 1 || !offs || !r || !w || !normals || !bfields || !centers ||
             !radii || !out)
@@ -621,11 +631,11 @@ int gimic_flux_planes(gimic_ctx *c, int nplanes, const int64_t *offs, const doub
         const size_t np = (size_t)std::max<int64_t>(npts, 1);
         c->pts_tmp.alloc(np * 24 + np * 8, st, false);  // points then weights
         c->jt_tmp.alloc(np * 72, st, false);
-        c->contrib.alloc(np * 16, st, true);  // current, then signed modulus
+        c->contrib.alloc(np * 24, st, true);  // current, signed modulus, ACID
         c->planes.alloc(np * 4, st, false);
         c->pparams.alloc(params.size() * 8, st, false);
         c->offs.alloc((size_t)(nplanes + 1) * 8, st, false);
-        c->outs.alloc((size_t)nplanes * 48, st, false);
+        c->outs.alloc((size_t)nplanes * 72, st, false);
         double *d_r = c->pts_tmp.as<double>(), *d_w = d_r + 3 * np;
         hip_check(hipMemcpyAsync(d_r, r, (size_t)npts * 24, hipMemcpyHostToDevice, st), "copy r");
         hip_check(hipMemcpyAsync(d_w, w, (size_t)npts * 8, hipMemcpyHostToDevice, st), "copy w");
@@ -645,6 +655,7 @@ int gimic_flux_planes(gimic_ctx *c, int nplanes, const int64_t *offs, const doub
         fd.pradius = pp + 9 * nplanes;
         fd.contrib = c->contrib.as<double>();
         fd.contrib_mod = out_mod ? c->contrib.as<double>() + np : nullptr;
+        fd.contrib_acid = out_acid ? c->contrib.as<double>() + 2 * np : nullptr;
         evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd);
         hip_check(launch_plane_reduce(c->contrib.as<double>(), (const long long *)c->offs.p,
                                       nplanes, c->outs.as<double>(), st),
@@ -659,6 +670,18 @@ int gimic_flux_planes(gimic_ctx *c, int nplanes, const int64_t *offs, const doub
             hip_check(hipMemcpyAsync(out_mod, c->outs.as<double>() + 3 * nplanes,
                                      (size_t)nplanes * 24, hipMemcpyDeviceToHost, st),
                       "copy out_mod");
+        }
+        if (out_acid) {
+            std::vector<double> tmp(3 * (size_t)nplanes);
+            hip_check(launch_plane_reduce(c->contrib.as<double>() + 2 * np,
+                                          (const long long *)c->offs.p, nplanes,
+                                          c->outs.as<double>() + 6 * nplanes, st),
+                      "plane_reduce");
+            hip_check(hipMemcpyAsync(tmp.data(), c->outs.as<double>() + 6 * nplanes,
+                                     (size_t)nplanes * 24, hipMemcpyDeviceToHost, st),
+                      "copy out_acid");
+            hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            for (int k = 0; k < nplanes; k++) out_acid[k] = tmp[3 * k];
         }
         if (jt)
             hip_check(hipMemcpyAsync(jt, c->jt_tmp.p, (size_t)npts * 72, hipMemcpyDeviceToHost,

@@ -101,6 +101,7 @@ struct FinalizeArgs {
     const double *pnorm, *pbfield, *pcenter, *pradius;  // per plane
     double *contrib;     // [npts] or null
     double *contrib_mod; // signed-modulus contribution w_p sgn(n.jv)|jv| or null
+    double *contrib_acid; // ACID contribution w_p dT^2 or null
 };
 hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s);
 

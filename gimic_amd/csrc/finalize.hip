@@ -10,7 +10,8 @@
 // The flux contribution of a point is w_p n.(J B) (integral.f90:119-143),
 // zero outside the integration radius around the plane centre; the signed
 // modulus contribution is w_p sgn(n.jv) |jv| with jv = J B and a 1e-12 dead
-// zone on the sign (integral.f90:258-283), from the same tensor; per-plane
+// zone on the sign (integral.f90:258-283), and the ACID density w_p dT^2
+// (acid.f90:11-40, integral.f90:414-511), all from the same tensor; per-plane
 // sums (net, positive part, negative part) use a fixed-order tree so results
 // are reproducible run to run.
 #include "device_common.h"
@@ -105,6 +106,14 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
                 }
                 const double sgn = fabs(dot) < 1e-12 ? 0.0 : (dot > 0.0 ? 1.0 : -1.0);
                 a.contrib_mod[p] = out ? 0.0 : sgn * sqrt(jv2) * a.w[p];
+            }
+            if (a.contrib_acid) {
+                // get_acid, acid.f90:11-40 (DP33 = 0.3333333 as in the reference)
+                const double d1 = J[0] - J[4], d2 = J[4] - J[8], d3 = J[8] - J[0];
+                const double o1 = J[3] + J[1], o2 = J[6] + J[2], o3 = J[7] + J[5];
+                const double dt2 = 0.3333333 * (d1 * d1 + d2 * d2 + d3 * d3) +
+                                   0.5 * (o1 * o1 + o2 * o2 + o3 * o3);
+                a.contrib_acid[p] = out ? 0.0 : dt2 * a.w[p];
             }
         }
     }

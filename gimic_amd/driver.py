@@ -5,10 +5,11 @@ MOL/XDENS files it names, builds the grid and field (grid.f90, magnet.f90)
 and runs
   calc=integral  signed-modulus flux when Essential.jmod is on, then the
                  current flux, per spin case (total; UHF adds alpha, beta,
-                 spindens) -- src/fgimic/gimic.F90:222-261, integral.f90
+                 spindens), then ACID when Essential.acid is on
+                 -- src/fgimic/gimic.F90:222-261, integral.f90
   calc=cdens     J B on every grid point (jfield.f90:62-184)
-printing the reference's stdout blocks.  Both integrals come from one tensor
-evaluation per spin case.
+printing the reference's stdout blocks.  All integrals come from one tensor
+evaluation per spin case (the reference evaluates J once per integral).
 
     python -m gimic_amd.driver gimic.inp [--precision fp64|mixed]
 """
@@ -49,8 +50,10 @@ def run_integral(kw, ctx, g, b, uhf, precision="fp64", out=sys.stdout):
     plane = flux.plane_of(g, b)
     res = {}
     for spin in spin_cases(uhf):
-        cur, mod = ctx.flux_planes([plane], spin=spin, precision=precision, modulus=True)
-        res[spin] = dict(current=tuple(cur[0]), modulus=tuple(mod[0]))
+        cur, mod, acid = ctx.flux_planes([plane], spin=spin, precision=precision, modulus=True,
+                                         acid=True)
+        res[spin] = dict(current=tuple(cur[0]), modulus=tuple(mod[0]),
+                         acid=float(np.sqrt(acid[0])))
     if jmod:
         print("*** Integrating |J|", file=out)
         for spin in spin_cases(uhf):
@@ -66,6 +69,13 @@ def run_integral(kw, ctx, g, b, uhf, precision="fp64", out=sys.stdout):
         if uhf:
             print(f"*** Integrating {spin if spin != 'spindens' else 'spin'} density", file=out)
         print(flux.report(*res[spin]["current"]), file=out)
+    if kw["Essential"].get("acid", False):
+        # integral.f90:500-510 (the reference integrates ACID for the total density)
+        a = res["total"]["acid"]
+        print("*" * 60, file=out)
+        print("   ACID (au) sqrt(delta J^2):%13.6f" % a, file=out)
+        print("   ACID (nA/T)              :%13.6f" % flux.au2si(a), file=out)
+        print("*" * 60, file=out)
     return res
 
 

@@ -21,7 +21,9 @@ EXPORTS = [
     "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
     "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
     "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_sph_transform", "gimic_quadrature",
-    "gimic_selftest_mfma_layout",
+    "gimic_selftest_mfma_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_info",
+    "gimic_mlp_params", "gimic_mlp_loss_grad", "gimic_mlp_train_step", "gimic_mlp_predict",
+    "gimic_mlp_predict_unfused",
 ]
 
 _lib = None
@@ -66,15 +68,23 @@ def load():
         "gimic_jtensor_batch": [P, I64, P, P, I, I],
         "gimic_jtensor_batch_device": [P, I64, P, P, I, I, P],
         "gimic_jtensor_multi_device": [P, I64, P, P, I, P, I, P],
-        "gimic_flux_planes": [P, I, P, P, P, P, P, P, P, I, I, P, P, P],
+        "gimic_flux_planes": [P, I, P, P, P, P, P, P, P, I, I, P, P, P, P],
         "gimic_sph_transform": [I, P],
         "gimic_quadrature": [D, D, I, I, I, P, P],
         "gimic_selftest_mfma_layout": [P, P],
+        "gimic_mlp_create": [P, D, P, P, ctypes.c_uint64],
+        "gimic_mlp_destroy": [P],
+        "gimic_mlp_info": [P, P, P],
+        "gimic_mlp_params": [P, P, I],
+        "gimic_mlp_loss_grad": [P, ctypes.c_longlong, P, P, D, P, P],
+        "gimic_mlp_train_step": [P, ctypes.c_longlong, P, P, D, D, P],
+        "gimic_mlp_predict": [P, ctypes.c_longlong, P, P, P],
+        "gimic_mlp_predict_unfused": [P, ctypes.c_longlong, P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
-        if name.startswith("gimic_ctx") or name in ("gimic_jtensor_batch", "gimic_jtensor_batch_device",
+        if name.startswith(("gimic_ctx", "gimic_mlp")) or name in ("gimic_jtensor_batch", "gimic_jtensor_batch_device",
                                                     "gimic_jtensor_multi_device", "gimic_flux_planes",
                                                     "gimic_sph_transform", "gimic_quadrature",
                                                     "gimic_selftest_mfma_layout"):
@@ -254,10 +264,12 @@ class Context:
         return out
 
     def flux_planes(self, planes, spin="total", precision="fp64", return_jt=False,
-                    modulus=False):
+                    modulus=False, acid=False):
         """planes: list of dicts with points (N,3), weights (N), normal, bfield,
         center, radius.  Returns (nplanes, 3): net, positive, negative (au);
-        with modulus=True also the signed-modulus sums (nplanes, 3)."""
+        with modulus=True also the signed-modulus sums (nplanes, 3), with
+        acid=True the ACID integrals sum w dT^2 (nplanes,), then the tensors
+        if return_jt."""
         offs = np.zeros(len(planes) + 1, dtype=np.int64)
         for k, p in enumerate(planes):
             offs[k + 1] = offs[k] + len(p["weights"])
@@ -269,11 +281,14 @@ class Context:
         rad = _f64([p.get("radius", 1e10) for p in planes])
         out = np.zeros((len(planes), 3))
         mod = np.zeros((len(planes), 3)) if modulus else None
+        aci = np.zeros(len(planes)) if acid else None
         jt = np.zeros((int(offs[-1]), 9)) if return_jt else None
         check(self._lib.gimic_flux_planes(self._h, len(planes), ptr(offs), ptr(r), ptr(w),
                                           ptr(nrm), ptr(bf), ptr(ctr), ptr(rad), SPIN[spin],
                                           PREC[precision], ptr(out),
                                           ptr(mod) if mod is not None else None,
+                                          ptr(aci) if aci is not None else None,
                                           ptr(jt) if jt is not None else None))
-        res = (out,) + ((mod,) if modulus else ()) + ((jt,) if return_jt else ())
+        res = (out,) + ((mod,) if modulus else ()) + ((aci,) if acid else ()) + \
+            ((jt,) if return_jt else ())
         return res if len(res) > 1 else out

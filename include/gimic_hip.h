@@ -153,13 +153,15 @@ int gimic_jtensor_multi_device(gimic_ctx *ctx, int64_t npts, const double *d_r, 
  * centre and radius bound (points farther than radius from the centre
  * contribute zero; pass 1e10 for none).  out[3*k..] = net, positive,
  * negative flux in au.  out_mod (optional, may be NULL) receives the same
- * three sums of the signed modulus of J B (integral.f90:190-324) from the
- * same tensors.  Host pointers; jt (optional, may be NULL) receives the
- * tensors of all points (9 per point). */
+ * three sums of the signed modulus of J B (integral.f90:190-324) and
+ * out_acid (optional) one sum per plane of w dT^2, the ACID integrand
+ * (acid.f90:11-40; the reference prints its square root, integral.f90:414-511),
+ * all from the same tensors.  Host pointers; jt (optional, may be NULL)
+ * receives the tensors of all points (9 per point). */
 int gimic_flux_planes(gimic_ctx *ctx, int nplanes, const int64_t *plane_offs, const double *r,
                       const double *w, const double *normals, const double *bfields,
                       const double *centers, const double *radii, int spin, int precision,
-                      double *out, double *out_mod, double *jt);
+                      double *out, double *out_mod, double *out_acid, double *jt);
 
 /* Real solid-harmonic transform for angular momentum l <= 5:
  * T[(m+l)*ncart + c], m = -l..l, cartesian components in the reference's
@@ -171,6 +173,42 @@ int gimic_sph_transform(int l, double *T);
  * (gaussint.f90:267-319): kind 0 Gauss-Legendre, 1 Gauss-Lobatto. */
 int gimic_quadrature(double a, double b, int order, int npts, int kind, double *pts,
                      double *wgts);
+
+/* ---------------------------------------------------------------------------
+ * Learned surrogate of the J field (no counterpart in the reference): a
+ * per-molecule coordinate network r -> 64 Fourier frequencies (sin, cos) + r
+ * -> 4 x 128 SiLU -> 9 tensor elements, trained by these entry points on
+ * tensors from the exact kernel (BF16 MFMA, FP32 master weights, Adam).
+ * Loss: mean over points of |y - t|^2 / (|t|^2 + eps^2), t = J / out_scale.
+ * Device pointers for points and tensors; the exact kernel stays the
+ * fallback and the judge (see gimic_amd/surrogate.py).
+ * ------------------------------------------------------------------------- */
+typedef struct gimic_mlp gimic_mlp;
+
+/* sigma: std-dev of the Fourier frequencies (cycles per unit of the scaled
+ * coordinate u = (r - centre)/scale); center_scale = {cx, cy, cz, scale};
+ * out_scale: 9 output scales; seed: initialisation. */
+int gimic_mlp_create(gimic_mlp **net, double sigma, const double *center_scale,
+                     const double *out_scale, uint64_t seed);
+int gimic_mlp_destroy(gimic_mlp *net);
+/* parameter count and the 3 x 64 frequency matrix (either may be NULL) */
+int gimic_mlp_info(const gimic_mlp *net, long long *nparams, float *freqs);
+/* copy the FP32 master parameters out (set = 0) or in (set = 1); layout per
+ * layer l: W_l[n_out][k_in] then b_l[n_out]; k_in 160 then 128, n_out 128,
+ * the output layer 16 rows (9 used) */
+int gimic_mlp_params(gimic_mlp *net, float *params, int set);
+/* loss and (optionally, host pointer) its gradient for one batch, no update */
+int gimic_mlp_loss_grad(gimic_mlp *net, long long npts, const double *d_r,
+                        const double *d_target, double eps, double *loss, float *grad);
+/* one Adam step on a batch; loss (host, may be NULL) before the update */
+int gimic_mlp_train_step(gimic_mlp *net, long long npts, const double *d_r,
+                         const double *d_target, double eps, double lr, double *loss);
+/* fused inference: d_J[9*i + m + 3*b] for npts points on `stream` (NULL: the
+ * network's stream) */
+int gimic_mlp_predict(gimic_mlp *net, long long npts, const double *d_r, double *d_J,
+                      void *stream);
+/* the same network evaluated with the training kernels (for checks) */
+int gimic_mlp_predict_unfused(gimic_mlp *net, long long npts, const double *d_r, double *d_J);
 
 /* Diagnostic (no counterpart in the reference): runs the two MFMA builtins
  * the kernels use on one wavefront with lane-numbered operands and returns

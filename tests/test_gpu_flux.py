@@ -82,3 +82,33 @@ def test_mixed_precision_flux_error_reported(tmp_path):
     print("ring12 plane flux fp64", f64, "mixed", mix, "rel err", rel)
     assert rel[0] <= 1e-6
     ctx.close()
+
+
+def test_acid_integral_matches_deviatoric_norm():
+    """ACID integrand dT^2 (acid.f90:11-40) equals |dev(sym J)|_F^2 up to the
+    reference's 0.3333333 constant; checked on the GPU tensors of a plane"""
+    from gimic_amd import flux, grid as gridmod, synth
+    s = synth.ring12()
+    ctx = s.context()
+    kw = {"_arg": "bond", "type": "gauss", "gauss_order": 9, "bond": [1, 2],
+          "fixcoord": [0.0, 0.0, 0.0], "distance": 1.3133, "height": [-4.0, 4.0],
+          "width": [-2.2, 9.8], "grid_points": [36, 54, 0], "radius": 5.0}
+    g = gridmod.new_grid(kw, np.asarray(s.coords))
+    B = gridmod.get_magnet(g, "X")
+    plane = flux.plane_of(g, B)
+    print("case acid", flush=True)
+    cur, acid, jt = ctx.flux_planes([plane], acid=True, return_jt=True)
+    J = jt.reshape(-1, 3, 3).transpose(0, 2, 1)       # J[p, m, b]
+    S = 0.5 * (J + J.transpose(0, 2, 1))
+    dev = S - np.einsum("pii->p", S)[:, None, None] * np.eye(3) / 3.0
+    dt2 = np.sum(dev * dev, axis=(1, 2))
+    w = np.where(np.linalg.norm(g.points() - g.center(), axis=1) > 5.0, 0.0, g.weights())
+    ref = np.sum(w * dt2)
+    assert abs(acid[0] - ref) <= 2e-7 * ref
+    # isotropic and antisymmetric tensors carry no ACID density
+    iso = np.eye(3) * 2.0
+    anti = np.array([[0, 1.0, -2.0], [-1.0, 0, 0.5], [2.0, -0.5, 0]])
+    for T in (iso, anti):
+        Sx = 0.5 * (T + T.T)
+        assert np.allclose(Sx - np.trace(Sx) / 3 * np.eye(3), 0) or np.allclose(Sx, 0)
+    ctx.close()
