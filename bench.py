@@ -10,6 +10,14 @@ contraction kernel's roofline position and the reference CPU code timed on
 this host are reported beside it.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Other workloads (one JSON line each; not the headline):
+    --workload scan      256 uneven Gauss-Lobatto planes across a bond of a
+                         30-atom sheet, one hipGraph launch per scan
+    --workload openshell 200-atom sheet, alpha/beta densities (8 matrices),
+                         charge and spin current on a 192x192x96 grid
+    --workload surrogate learned J field of the 60-atom cage on a 256^3 grid
+                         with exact-kernel fallback
 """
 import argparse
 import json
@@ -116,6 +124,162 @@ def cpu_baseline(system, budget_s=15.0):
                       f"density D_cart = T^T D T, loop time {t:.2f} s"}
 
 
+def scan_planes(system, nplanes=256, seed=7):
+    """planes perpendicular to the bond of atoms 1-2, offsets -6..+6 bohr from
+    the bond midpoint, each with its own order-7 Lobatto grid of n1 x n2
+    points, n1 in [40, 120], n2 in [60, 200] (seeded, rounded up to the order)"""
+    from gimic_amd import flux, grid as gridmod
+    c = np.asarray(system.coords)
+    half = float(np.linalg.norm(c[1] - c[0]) / 2)
+    rng = np.random.default_rng(seed)
+    planes = []
+    for off in np.linspace(-6.0, 6.0, nplanes):
+        n1, n2 = int(rng.integers(40, 121)), int(rng.integers(60, 201))
+        kw = {"_arg": "bond", "type": "lobatto", "gauss_order": 7, "bond": [1, 2],
+              "fixcoord": [0.0, 0.0, 10.0], "distance": half + float(off),
+              "height": [-8.0, 8.0], "width": [-10.0, 10.0], "grid_points": [n1, n2, 0]}
+        g = gridmod.new_grid(kw, c)
+        planes.append(flux.plane_of(g, gridmod.get_magnet(g, "X")))
+    return planes
+
+
+def bench_scan(args):
+    import torch
+    from gimic_amd import synth
+    system = synth.sheet30()
+    ctx = system.context(screening=1e-8)
+    planes = scan_planes(system)
+    npts = sum(len(p["weights"]) for p in planes)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        out = {}
+        for mode in ("eager", "graph"):
+            ctx.set_graph(mode == "graph")
+            for _ in range(args.warmup):
+                ctx.flux_planes(planes, precision=prec, modulus=True)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                f, m = ctx.flux_planes(planes, precision=prec, modulus=True)
+            out[mode] = (time.perf_counter() - t0) / args.steps
+        res[prec] = dict(flux=f, ms=out)
+    ctx.close()
+    f64, fmx = res["fp64"]["flux"], res["mixed"]["flux"]
+    rel = np.abs(fmx[:, 0] - f64[:, 0]) / np.maximum(np.abs(f64[:, 0]), 1e-300)
+    big = np.abs(f64[:, 0]) >= 1e-3 * np.abs(f64[:, 0]).max()
+    t = res["mixed"]["ms"]["graph"]
+    return {
+        "metric": "plane-scan J-tensor points/s (one hipGraph launch per scan)",
+        "value": round(npts / t, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t, 3), "higher_is_better": True,
+        "dtype": "fp32 (bf16x6 MFMA), fp64 assembly", "data": "synthetic",
+        "config": {"workload": "sheet30 tetracene C18H12, nbf 312, nocc 60; 256 planes "
+                               "perpendicular to bond 1-2, offsets -6..+6 bohr, order-7 "
+                               "Lobatto, 40..120 x 60..200 points per plane",
+                   "points_per_step": int(npts), "planes": len(planes)},
+        "wall_ms": {p: {m: round(1e3 * v, 3) for m, v in res[p]["ms"].items()} for p in res},
+        "mixed_over_fp64": round(res["fp64"]["ms"]["graph"] / t, 3),
+        "error": {"flux_rel_max": float(rel[big].max()), "flux_rel_median": float(np.median(rel)),
+                  "planes_checked": int(big.sum())},
+    }
+
+
+def bench_openshell(args):
+    import torch
+    from gimic_amd import synth
+    system = synth.sheet200()
+    ctx = system.context(screening=1e-8)
+    pts = np.stack(np.meshgrid(0.2 * np.arange(192) - 19.1, 0.2 * np.arange(192) - 19.1,
+                               0.2 * np.arange(96) - 9.5, indexing="ij"), -1)
+    pts = pts.transpose(2, 1, 0, 3).reshape(-1, 3)  # x fastest
+    d_pts = torch.tensor(pts, dtype=torch.float64, device="cuda")
+    npts = len(pts)
+    spins = ["total", "spindens"]  # charge and spin current, 8 matrices
+    out = torch.empty((2, npts, 9), dtype=torch.float64, device="cuda")
+    ms = {}
+    for prec in ("mixed", "fp64"):
+        for _ in range(args.warmup):
+            ctx.jtensor_multi_device(d_pts, spins, precision=prec, out=out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.jtensor_multi_device(d_pts, spins, precision=prec, out=out)
+        torch.cuda.synchronize()
+        ms[prec] = 1e3 * (time.perf_counter() - t0) / args.steps
+        if prec == "mixed":
+            Jm = out.clone()
+    err = {}
+    for k, name in enumerate(("charge", "spin")):
+        d = torch.linalg.norm(Jm[k] - out[k], dim=1)
+        nb = torch.linalg.norm(out[k], dim=1)
+        e = (d / torch.clamp(nb, min=1e-3 * float(nb.max()))).cpu().numpy()
+        err[name] = {"tensor_rel_max": float(e.max()), "tensor_rel_p99": float(np.quantile(e, 0.99)),
+                     "field_rel": float(torch.linalg.norm(Jm[k] - out[k]) / torch.linalg.norm(out[k]))}
+    ctx.close()
+    return {
+        "metric": "open-shell J-tensor points/s (charge + spin, 8 matrices)",
+        "value": round(npts / (ms["mixed"] / 1e3), 1), "unit": "points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms["mixed"], 3),
+        "higher_is_better": True, "dtype": "fp32 (bf16x6 MFMA), fp64 assembly",
+        "data": "synthetic",
+        "config": {"workload": "sheet200 C166H34, nbf 2494, alpha/beta nocc 610/600, "
+                               "192x192x96 grid at 0.2 bohr", "points_per_step": npts,
+                   "nbf": system.nbf},
+        "fp64_gpu": {"value": round(npts / (ms["fp64"] / 1e3), 1), "ms_per_step": round(ms["fp64"], 3)},
+        "mixed_over_fp64": round(ms["fp64"] / ms["mixed"], 3),
+        "error": err,
+    }
+
+
+def bench_surrogate(args):
+    import torch
+    from gimic_amd import synth
+    from gimic_amd.surrogate import Surrogate
+    system = synth.cage60()
+    ctx = system.context(screening=1e-8)
+    half = 7.68
+    sg = Surrogate(ctx, np.asarray(system.coords), -half * np.ones(3), half * np.ones(3),
+                   tol=1e-3, ncell=8, sigma=args.sigma, seed=3)
+    st = sg.fit(n_train=2_000_000, steps=args.train_steps, batch=16384, lr=2e-3)
+    st.pop("loss_history", None)
+    ax = torch.linspace(-half, half, 256, dtype=torch.float64, device="cuda")
+    z, y, x = torch.meshgrid(ax, ax, ax, indexing="ij")
+    grid = torch.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    out = torch.empty((grid.shape[0], 9), dtype=torch.float64, device="cuda")
+    sg.evaluate(grid, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, served = sg.evaluate(grid, out=out)
+    torch.cuda.synchronize()
+    t_h = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sg.net.predict(grid, out=out)
+    torch.cuda.synchronize()
+    t_net = time.perf_counter() - t0
+    J, served = sg.evaluate(grid, out=out)
+    judge = sg.judge(grid, J, served)
+    sub = grid[::8].contiguous()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.jtensor_device(sub, precision="mixed")
+    torch.cuda.synchronize()
+    t_ex = (time.perf_counter() - t0) * 8
+    ctx.close()
+    n = grid.shape[0]
+    return {
+        "metric": "visualisation-grid J-tensor points/s with learned surrogate + exact fallback",
+        "value": round(n / t_h, 1), "unit": "points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(1e3 * t_h, 3), "higher_is_better": True,
+        "dtype": "bf16 MFMA (fp32 master weights); fallback fp32 (bf16x6 MFMA)",
+        "data": "synthetic",
+        "config": {"workload": "cage60 (nbf 1800) on a 256^3 grid over [-7.68, 7.68]^3 bohr; "
+                               "2M exact-kernel training points, 1% held out, 8^3 cells",
+                   "points_per_step": n},
+        "served_fraction": float(served.float().mean()),
+        "exact_only_points_per_s": round(n / t_ex, 1), "network_only_points_per_s": round(n / t_net, 1),
+        "gain_over_exact": round(t_ex / t_h, 3), "training": st, "judge": judge,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +287,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
     ap.add_argument("--points", type=int, default=0, help="limit points (profiling)")
+    ap.add_argument("--workload", default="cage60",
+                    choices=["cage60", "scan", "openshell", "surrogate"])
+    ap.add_argument("--train-steps", type=int, default=6000, help="surrogate training steps")
+    ap.add_argument("--sigma", type=float, default=3.0, help="surrogate Fourier frequency scale")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the workload is sharded by point tiles "
@@ -131,6 +299,10 @@ def main():
     import torch
     from gimic_amd import flux, synth
     assert torch.cuda.is_available(), "no GPU"
+    if args.workload != "cage60":
+        fn = {"scan": bench_scan, "openshell": bench_openshell, "surrogate": bench_surrogate}
+        print(json.dumps(fn[args.workload](args)))
+        return
     dev = torch.device("cuda:0")
     log("setup cage60")
     system = synth.cage60()

@@ -120,8 +120,16 @@ struct gimic_ctx {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
     double ms[3] = {0.0, 0.0, 0.0};
     long long launches[3] = {0, 0, 0};
+    // flux planes as one captured hipGraph launch, reused while its key
+    // (sizes, flags, kernel choice and every device address it bakes in)
+    // matches
+    bool graphs = false;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uintptr_t> graph_key;
+    long long graph_builds = 0, graph_launches = 0;
 
     ~gimic_ctx() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (auto &v : ev)
             for (auto &e : v) {
                 (void)hipEventDestroy(e.first);
@@ -279,32 +287,59 @@ struct FluxDev {
     double *contrib_acid = nullptr;
 };
 
+enum class EvalMode {
+    Eager,    // allocate as needed, tune, launch
+    Reserve,  // allocate every buffer the launches need and tune; launch nothing else
+    Capture,  // kernel launches only (no allocation, memset, timing or sync)
+};
+
 // Evaluate J for npts device points into d_jt (nspin blocks of 9*npts).
 void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
-              const std::vector<int> &spins, int prec, hipStream_t st, const FluxDev *flux) {
+              const std::vector<int> &spins, int prec, hipStream_t st, const FluxDev *flux,
+              EvalMode mode = EvalMode::Eager) {
     if (npts <= 0) return;
     if (prec != GIMIC_PREC_FP64 && prec != GIMIC_PREC_MIXED)
         throw Error(GIMIC_EINVAL, "unknown precision");
     for (int s : spins)
         if (s < 0 || s > 3) throw Error(GIMIC_EINVAL, "invalid spin case");
+    const bool capture = mode == EvalMode::Capture;
+    if (capture && !c->mats.count(std::make_pair(spins, prec)))
+        throw Error(GIMIC_ESTATE, "capture before reserve: right operands missing");
     auto &M = get_mats(c, spins, prec);
+    // per-kernel event timing only for eager evaluation
+    struct TimingOff {
+        gimic_ctx *c;
+        bool saved;
+        ~TimingOff() { c->timing = saved; }
+    } timing_guard{c, c->timing};
+    if (mode != EvalMode::Eager) c->timing = false;
+    auto grow = [&](DevBuf &b, size_t bytes, bool zero) {
+        if (capture) {
+            if (b.bytes < bytes || !b.p) throw Error(GIMIC_ESTATE, "capture before reserve");
+            return;
+        }
+        b.alloc(bytes, st, zero);
+    };
+
     const size_t ld = (size_t)c->lda;
     // chunk of points per pass: basis values + gradients of one chunk stay
     // within ~4 GiB (FP64: 32 bytes per point and function)
     const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 16);
-    // row granularity: 128 for the FP64 kernel, 256 for the mixed kernels
+    // row granularity: the workgroup height of the contraction kernels (128)
     const size_t gran = prec == GIMIC_PREC_FP64 ? (size_t)kWgRows : (size_t)contract_mx_rows();
     size_t cap = (size_t)4 << 30;
     size_t chunk = std::max<size_t>(gran, (cap / per_row) / gran * gran);
     chunk = std::min(chunk, round_up((size_t)npts, gran));
     const size_t rows_cap = chunk;
     const long long gplane = (long long)(rows_cap * ld);
+    // padding columns and rows of the basis buffers must read as zero: they
+    // are cleared when (re)allocated and never written afterwards
     if (prec == GIMIC_PREC_FP64) {
-        c->phi64.alloc(rows_cap * ld * 8, st, c->phi64.bytes < rows_cap * ld * 8);
-        c->grad64.alloc(3 * rows_cap * ld * 8, st, c->grad64.bytes < 3 * rows_cap * ld * 8);
+        grow(c->phi64, rows_cap * ld * 8, c->phi64.bytes < rows_cap * ld * 8);
+        grow(c->grad64, 3 * rows_cap * ld * 8, c->grad64.bytes < 3 * rows_cap * ld * 8);
     } else {
-        c->phi32.alloc(rows_cap * ld * 4, st, c->phi32.bytes < rows_cap * ld * 4);
-        c->grad32.alloc(3 * rows_cap * ld * 4, st, c->grad32.bytes < 3 * rows_cap * ld * 4);
+        grow(c->phi32, rows_cap * ld * 4, c->phi32.bytes < rows_cap * ld * 4);
+        grow(c->grad32, 3 * rows_cap * ld * 4, c->grad32.bytes < 3 * rows_cap * ld * 4);
     }
     const int nmatset = M.nmatset;
     for (size_t start = 0; start < (size_t)npts; start += chunk) {
@@ -328,9 +363,6 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         ea.grad64 = c->grad64.as<double>();
         ea.phi32 = c->phi32.as<float>();
         ea.grad32 = c->grad32.as<float>();
-        timed(c, 0, st, [&] {
-            hip_check(launch_basis_eval(ea, prec == GIMIC_PREC_FP64 ? 0 : 1, st), "basis_eval");
-        });
 
         ContractArgs ca{};
         ca.rows = rows;
@@ -356,48 +388,57 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
                                                   std::max<long long>(1, (2048 + per - 1) / per));
             ca.tiles_per_split = (ca.ntiles + nsplit - 1) / nsplit;
             ca.nsplit = (ca.ntiles + ca.tiles_per_split - 1) / ca.tiles_per_split;
-            c->part.alloc((size_t)nmatset * ca.nsplit * kNQ * rows * 8, st, false);
+            grow(c->part, (size_t)nmatset * ca.nsplit * kNQ * rows * 8, false);
             ca.part = c->part.as<double>();
         };
-        auto run = [&](int variant) {
-            if (variant == 0) {
-                plan(kWgRows, kNT);
-                hip_check(launch_contract_f64(ca, st), "contract_f64");
-            } else if (variant == 1) {
-                plan(kWgRows, kNT);
-                hip_check(launch_contract_bf16x6(ca, st), "contract_bf16x6");
-            } else {
-                plan(contract_mx_rows(), contract_mx_cols());
-                hip_check(launch_contract_mx(ca, st), "contract_mx");
-            }
+        auto plan_variant = [&](int variant) {
+            if (variant == 2) plan(contract_mx_rows(), contract_mx_cols());
+            else plan(kWgRows, kNT);
         };
-        if (prec == GIMIC_PREC_FP64) {
-            timed(c, 1, st, [&] { run(0); });
-        } else {
-            if (c->mixed_variant < 0) {
-                // autotune once per context: time both mixed kernels on this chunk
-                float best = 0.f;
-                for (int v = 1; v <= 2; v++) {
-                    hipEvent_t e0, e1;
-                    hip_check(hipEventCreate(&e0), "hipEventCreate");
-                    hip_check(hipEventCreate(&e1), "hipEventCreate");
-                    hip_check(hipEventRecord(e0, st), "hipEventRecord");
-                    run(v);
-                    hip_check(hipEventRecord(e1, st), "hipEventRecord");
-                    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-                    float ms = 0.f;
-                    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-                    (void)hipEventDestroy(e0);
-                    (void)hipEventDestroy(e1);
-                    c->tune_ms[v - 1] = ms;
-                    if (c->mixed_variant < 0 || ms < best) {
-                        best = ms;
-                        c->mixed_variant = v;
-                    }
+        auto run = [&](int variant) {
+            plan_variant(variant);
+            if (variant == 0) hip_check(launch_contract_f64(ca, st), "contract_f64");
+            else if (variant == 1) hip_check(launch_contract_bf16x6(ca, st), "contract_bf16x6");
+            else hip_check(launch_contract_mx(ca, st), "contract_mx");
+        };
+        const bool tune = prec == GIMIC_PREC_MIXED && c->mixed_variant < 0;
+        if (capture && tune) throw Error(GIMIC_ESTATE, "mixed kernel not tuned before capture");
+        if (mode == EvalMode::Reserve && !tune) {
+            plan_variant(prec == GIMIC_PREC_FP64 ? 0 : c->mixed_variant);
+            continue;
+        }
+        timed(c, 0, st, [&] {
+            hip_check(launch_basis_eval(ea, prec == GIMIC_PREC_FP64 ? 0 : 1, st), "basis_eval");
+        });
+        if (tune) {
+            // autotune once per context: time both mixed kernels on this chunk
+            float best = 0.f;
+            for (int v = 1; v <= 2; v++) {
+                hipEvent_t e0, e1;
+                hip_check(hipEventCreate(&e0), "hipEventCreate");
+                hip_check(hipEventCreate(&e1), "hipEventCreate");
+                plan_variant(v);  // allocate before the timed region
+                hip_check(hipEventRecord(e0, st), "hipEventRecord");
+                run(v);
+                hip_check(hipEventRecord(e1, st), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                c->tune_ms[v - 1] = ms;
+                if (c->mixed_variant < 0 || ms < best) {
+                    best = ms;
+                    c->mixed_variant = v;
                 }
             }
-            timed(c, 1, st, [&] { run(c->mixed_variant); });
+            if (mode == EvalMode::Reserve) {
+                plan_variant(c->mixed_variant);
+                continue;
+            }
         }
+        const int variant = prec == GIMIC_PREC_FP64 ? 0 : c->mixed_variant;
+        timed(c, 1, st, [&] { run(variant); });
         const int nsplit = ca.nsplit;
 
         FinalizeArgs fa{};
@@ -656,37 +697,86 @@ int gimic_flux_planes(gimic_ctx *c, int nplanes, const int64_t *offs, const doub
         fd.contrib = c->contrib.as<double>();
         fd.contrib_mod = out_mod ? c->contrib.as<double>() + np : nullptr;
         fd.contrib_acid = out_acid ? c->contrib.as<double>() + 2 * np : nullptr;
-        evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd);
-        hip_check(launch_plane_reduce(c->contrib.as<double>(), (const long long *)c->offs.p,
-                                      nplanes, c->outs.as<double>(), st),
-                  "plane_reduce");
-        hip_check(hipMemcpyAsync(out, c->outs.p, (size_t)nplanes * 24, hipMemcpyDeviceToHost, st),
+        auto reduce = [&] {
+            for (int k = 0; k < 3; k++) {
+                if ((k == 1 && !out_mod) || (k == 2 && !out_acid)) continue;
+                hip_check(launch_plane_reduce(c->contrib.as<double>() + k * np,
+                                              (const long long *)c->offs.p, nplanes,
+                                              c->outs.as<double>() + 3 * k * nplanes, st),
+                          "plane_reduce");
+            }
+        };
+        if (!c->graphs) {
+            evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd);
+            reduce();
+        } else {
+            // all planes, every chunk and the per-plane reductions as one graph
+            evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd,
+                     EvalMode::Reserve);
+            auto &M = get_mats(c, {spin}, precision);
+            std::vector<uintptr_t> key = {
+                (uintptr_t)npts, (uintptr_t)nplanes, (uintptr_t)precision, (uintptr_t)spin,
+                (uintptr_t)(out_mod != nullptr), (uintptr_t)(out_acid != nullptr),
+                (uintptr_t)c->giao, (uintptr_t)c->diamag, (uintptr_t)c->paramag,
+                (uintptr_t)c->mixed_variant, (uintptr_t)st, (uintptr_t)c->pts_tmp.p,
+                (uintptr_t)c->jt_tmp.p, (uintptr_t)c->contrib.p, (uintptr_t)c->planes.p,
+                (uintptr_t)c->pparams.p, (uintptr_t)c->offs.p, (uintptr_t)c->outs.p,
+                (uintptr_t)c->phi64.p, (uintptr_t)c->grad64.p, (uintptr_t)c->phi32.p,
+                (uintptr_t)c->grad32.p, (uintptr_t)c->part.p, (uintptr_t)M.b64.p,
+                (uintptr_t)M.b32.p};
+            if (!c->graph_exec || key != c->graph_key) {
+                if (c->graph_exec) {
+                    hip_check(hipGraphExecDestroy(c->graph_exec), "hipGraphExecDestroy");
+                    c->graph_exec = nullptr;
+                }
+                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal),
+                          "hipStreamBeginCapture");
+                hipGraph_t g = nullptr;
+                try {
+                    evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd,
+                             EvalMode::Capture);
+                    reduce();
+                } catch (...) {
+                    (void)hipStreamEndCapture(st, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                hip_check(hipStreamEndCapture(st, &g), "hipStreamEndCapture");
+                hipError_t e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                hip_check(e, "hipGraphInstantiate");
+                c->graph_key = key;
+                c->graph_builds++;
+            }
+            hip_check(hipGraphLaunch(c->graph_exec, st), "hipGraphLaunch");
+            c->graph_launches++;
+        }
+        std::vector<double> res(9 * (size_t)nplanes);
+        hip_check(hipMemcpyAsync(res.data(), c->outs.p, res.size() * 8, hipMemcpyDeviceToHost, st),
                   "copy out");
-        if (out_mod) {
-            hip_check(launch_plane_reduce(c->contrib.as<double>() + np,
-                                          (const long long *)c->offs.p, nplanes,
-                                          c->outs.as<double>() + 3 * nplanes, st),
-                      "plane_reduce");
-            hip_check(hipMemcpyAsync(out_mod, c->outs.as<double>() + 3 * nplanes,
-                                     (size_t)nplanes * 24, hipMemcpyDeviceToHost, st),
-                      "copy out_mod");
-        }
-        if (out_acid) {
-            std::vector<double> tmp(3 * (size_t)nplanes);
-            hip_check(launch_plane_reduce(c->contrib.as<double>() + 2 * np,
-                                          (const long long *)c->offs.p, nplanes,
-                                          c->outs.as<double>() + 6 * nplanes, st),
-                      "plane_reduce");
-            hip_check(hipMemcpyAsync(tmp.data(), c->outs.as<double>() + 6 * nplanes,
-                                     (size_t)nplanes * 24, hipMemcpyDeviceToHost, st),
-                      "copy out_acid");
-            hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-            for (int k = 0; k < nplanes; k++) out_acid[k] = tmp[3 * k];
-        }
         if (jt)
             hip_check(hipMemcpyAsync(jt, c->jt_tmp.p, (size_t)npts * 72, hipMemcpyDeviceToHost,
                                      st), "copy jt");
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        std::copy(res.begin(), res.begin() + 3 * nplanes, out);
+        if (out_mod) std::copy(res.begin() + 3 * nplanes, res.begin() + 6 * nplanes, out_mod);
+        if (out_acid)
+            for (int k = 0; k < nplanes; k++) out_acid[k] = res[6 * (size_t)nplanes + 3 * k];
+    });
+}
+
+int gimic_ctx_set_graph(gimic_ctx *c, int enable) {
+    return guarded([&] {
+        if (!c) throw Error(GIMIC_EINVAL, "null context");
+        c->graphs = enable != 0;
+    });
+}
+
+int gimic_ctx_graph_stats(const gimic_ctx *c, long long *builds, long long *launches) {
+    return guarded([&] {
+        if (!c || !builds || !launches) throw Error(GIMIC_EINVAL, "null argument");
+        *builds = c->graph_builds;
+        *launches = c->graph_launches;
     });
 }
 

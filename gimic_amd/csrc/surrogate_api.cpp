@@ -75,7 +75,8 @@ struct Buf {
 }  // namespace
 
 struct gimic_mlp {
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;     // own stream
+    hipStream_t user = nullptr;       // caller's stream, when set
     long long np = 0;                 // parameter count
     long long woff[kD + 1], boff[kD + 1];
     int kin[kD + 1], nout[kD + 1];
@@ -94,6 +95,8 @@ struct gimic_mlp {
 namespace {
 
 std::string &errstr() { return error_slot(); }
+
+hipStream_t work_stream(const gimic_mlp *n) { return n->user ? n->user : n->stream; }
 
 template <class F>
 int guard(F &&f) {
@@ -207,7 +210,7 @@ int gimic_mlp_create(gimic_mlp **out, double sigma, const double *center_scale,
                            (float)center_scale[3]};
             float os[9];
             for (int i = 0; i < 9; i++) os[i] = (float)out_scale[i];
-            hipStream_t s = n->stream;
+            hipStream_t s = work_stream(n);
             n->params.need(off * 4);
             n->params_b.need(off * 2);
             n->grad.need(off * 4);
@@ -246,6 +249,14 @@ int gimic_mlp_destroy(gimic_mlp *n) {
     return guard([&] { delete n; });
 }
 
+int gimic_mlp_set_stream(gimic_mlp *n, void *stream) {
+    return guard([&] {
+        if (!n) throw MlpError(GIMIC_EINVAL, "null network");
+        chk(hipStreamSynchronize(work_stream(n)), "sync");
+        n->user = (hipStream_t)stream;
+    });
+}
+
 int gimic_mlp_info(const gimic_mlp *n, long long *nparams, float *freqs) {
     return guard([&] {
         if (!n) throw MlpError(GIMIC_EINVAL, "null network");
@@ -257,7 +268,7 @@ int gimic_mlp_info(const gimic_mlp *n, long long *nparams, float *freqs) {
 int gimic_mlp_params(gimic_mlp *n, float *host, int set) {
     return guard([&] {
         if (!n || !host) throw MlpError(GIMIC_EINVAL, "null argument");
-        hipStream_t s = n->stream;
+        hipStream_t s = work_stream(n);
         if (set) {
             chk(hipMemcpyAsync(n->params.p, host, n->np * 4, hipMemcpyHostToDevice, s), "copy");
             chk(mlp_cast(n->params.as<float>(), n->params_b.as<__bf16>(), n->np, s), "cast");
@@ -272,7 +283,7 @@ int gimic_mlp_loss_grad(gimic_mlp *n, long long npts, const double *d_r, const d
                         double eps, double *loss, float *host_grad) {
     return guard([&] {
         if (!n || npts <= 0 || !d_r || !d_target) throw MlpError(GIMIC_EINVAL, "bad argument");
-        hipStream_t s = n->stream;
+        hipStream_t s = work_stream(n);
         reserve(n, npts);
         forward(n, npts, d_r, s);
         backward(n, npts, d_target, (float)(eps * eps), s);
@@ -289,7 +300,7 @@ int gimic_mlp_train_step(gimic_mlp *n, long long npts, const double *d_r, const 
                          double eps, double lr, double *loss) {
     return guard([&] {
         if (!n || npts <= 0 || !d_r || !d_target) throw MlpError(GIMIC_EINVAL, "bad argument");
-        hipStream_t s = n->stream;
+        hipStream_t s = work_stream(n);
         reserve(n, npts);
         forward(n, npts, d_r, s);
         backward(n, npts, d_target, (float)(eps * eps), s);
@@ -311,7 +322,7 @@ int gimic_mlp_train_step(gimic_mlp *n, long long npts, const double *d_r, const 
 int gimic_mlp_predict(gimic_mlp *n, long long npts, const double *d_r, double *d_J, void *stream) {
     return guard([&] {
         if (!n || npts < 0 || (npts > 0 && (!d_r || !d_J))) throw MlpError(GIMIC_EINVAL, "bad argument");
-        hipStream_t s = stream ? (hipStream_t)stream : n->stream;
+        hipStream_t s = stream ? (hipStream_t)stream : work_stream(n);
         const __bf16 *const *W = (const __bf16 *const *)n->ptrs.p;
         const float *const *B = (const float *const *)((void **)n->ptrs.p + kD + 1);
         chk(mlp_infer(d_r, npts, n->freq.as<float>(), kF, n->cs.as<float>(), W, B,
@@ -322,7 +333,7 @@ int gimic_mlp_predict(gimic_mlp *n, long long npts, const double *d_r, double *d
 int gimic_mlp_predict_unfused(gimic_mlp *n, long long npts, const double *d_r, double *d_J) {
     return guard([&] {
         if (!n || npts <= 0 || !d_r || !d_J) throw MlpError(GIMIC_EINVAL, "bad argument");
-        hipStream_t s = n->stream;
+        hipStream_t s = work_stream(n);
         reserve(n, npts);
         forward(n, npts, d_r, s);
         chk(mlp_output(n->Z[kD].as<float>(), n->params.as<float>() + n->boff[kD],

@@ -19,9 +19,9 @@ EXPORTS = [
     "gimic_set_screening", "gimic_calc_jtensor", "gimic_calc_jvector", "gimic_calc_modj",
     "gimic_get_gauss_points", "mkgausspoints", "gimic_last_error", "gimic_ctx_create_files",
     "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
-    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
+    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
     "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_sph_transform", "gimic_quadrature",
-    "gimic_selftest_mfma_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_info",
+    "gimic_selftest_mfma_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_set_stream", "gimic_mlp_info",
     "gimic_mlp_params", "gimic_mlp_loss_grad", "gimic_mlp_train_step", "gimic_mlp_predict",
     "gimic_mlp_predict_unfused",
 ]
@@ -40,6 +40,16 @@ def load():
         return _lib
     if not os.path.exists(LIBPATH):
         raise GimicError(f"{LIBPATH} not found: build it with `python -m gimic_amd.build`")
+    # PyTorch's ROCm wheels bundle their own libamdhip64.so.7 and
+    # libhsa-runtime64.so.  Loading this library first would bind the system
+    # HIP/HSA runtime, and a later `import torch` would bring a second HSA
+    # runtime into the process, which then finds no device.  Importing torch
+    # first makes this library bind torch's runtime (same soname), so the
+    # process holds one HIP runtime and device pointers pass between the two.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIBPATH)
     P, I, D, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
     L.gimic_last_error.restype = ctypes.c_char_p
@@ -65,6 +75,8 @@ def load():
         "gimic_ctx_set_timing": [P, I],
         "gimic_ctx_get_timing": [P, P, P],
         "gimic_ctx_tuning": [P, P, P],
+        "gimic_ctx_set_graph": [P, I],
+        "gimic_ctx_graph_stats": [P, P, P],
         "gimic_jtensor_batch": [P, I64, P, P, I, I],
         "gimic_jtensor_batch_device": [P, I64, P, P, I, I, P],
         "gimic_jtensor_multi_device": [P, I64, P, P, I, P, I, P],
@@ -74,6 +86,7 @@ def load():
         "gimic_selftest_mfma_layout": [P, P],
         "gimic_mlp_create": [P, D, P, P, ctypes.c_uint64],
         "gimic_mlp_destroy": [P],
+        "gimic_mlp_set_stream": [P, P],
         "gimic_mlp_info": [P, P, P],
         "gimic_mlp_params": [P, P, I],
         "gimic_mlp_loss_grad": [P, ctypes.c_longlong, P, P, D, P, P],
@@ -204,6 +217,16 @@ class Context:
         ms = np.zeros(2)
         check(self._lib.gimic_ctx_tuning(self._h, ctypes.byref(v), ptr(ms)))
         return v.value, ms.tolist()
+
+    def set_graph(self, enable=True):
+        """capture flux_planes batches into one reusable hipGraph launch"""
+        check(self._lib.gimic_ctx_set_graph(self._h, 1 if enable else 0))
+
+    def graph_stats(self):
+        """(graph builds, graph launches) of flux_planes in graph mode"""
+        b, n = ctypes.c_longlong(), ctypes.c_longlong()
+        check(self._lib.gimic_ctx_graph_stats(self._h, ctypes.byref(b), ctypes.byref(n)))
+        return b.value, n.value
 
     def atoms(self):
         """(coords (natoms,3) bohr, charges (natoms,))"""

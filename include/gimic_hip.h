@@ -130,6 +130,15 @@ int gimic_ctx_get_timing(gimic_ctx *ctx, double *ms, long long *launches);
  * one.  Returns the choice (-1 before the first mixed call) and both times. */
 int gimic_ctx_tuning(const gimic_ctx *ctx, int *variant, double *ms);
 
+/* Graph mode for gimic_flux_planes (no counterpart in the reference): when
+ * enabled, the whole plane batch (basis evaluation, contraction and tensor
+ * assembly for every chunk, and the per-plane reductions) is captured once
+ * into a hipGraph and launched as one graph; later calls with the same
+ * sizes, flags and work buffers relaunch it without capturing again.
+ * graph_stats returns the number of graph builds and launches. */
+int gimic_ctx_set_graph(gimic_ctx *ctx, int enable);
+int gimic_ctx_graph_stats(const gimic_ctx *ctx, long long *builds, long long *launches);
+
 int gimic_ctx_destroy(gimic_ctx *ctx);
 
 /* J tensors at npts points: r[3*i..] (bohr) -> jt[9*i + m + 3*b].
@@ -191,6 +200,9 @@ typedef struct gimic_mlp gimic_mlp;
 int gimic_mlp_create(gimic_mlp **net, double sigma, const double *center_scale,
                      const double *out_scale, uint64_t seed);
 int gimic_mlp_destroy(gimic_mlp *net);
+/* run the network's kernels on `stream` (e.g. the caller's current stream),
+ * NULL: on the network's own stream; waits for the previous stream first */
+int gimic_mlp_set_stream(gimic_mlp *net, void *stream);
 /* parameter count and the 3 x 64 frequency matrix (either may be NULL) */
 int gimic_mlp_info(const gimic_mlp *net, long long *nparams, float *freqs);
 /* copy the FP32 master parameters out (set = 0) or in (set = 1); layout per

@@ -1,0 +1,141 @@
+"""Learned surrogate on the GPU: forward pass and gradients of the HIP
+training kernels against the CPU oracle (oracle/mlp_ref.py), training on
+exact-kernel tensors, and the exact-kernel fallback."""
+import numpy as np
+import pytest
+
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+CS = np.array([0.0, 0.0, 0.0, 6.0])
+
+
+def _setup(n=4096, seed=0):
+    import torch
+    from gimic_amd import synth
+    s = synth.ring12()
+    ctx = s.context()
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    r = (torch.rand((n, 3), generator=g, dtype=torch.float64, device="cuda") - 0.5) * 12.0
+    J = ctx.jtensor_device(r, precision="fp64")
+    rms = torch.sqrt((J ** 2).mean(dim=0)).cpu().numpy()
+    return s, ctx, r, J, rms
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_forward_matches_oracle():
+    from gimic_amd.surrogate import Network
+    s, ctx, r, J, rms = _setup()
+    net = Network(2.0, CS, rms, seed=1)
+    print("case mlp forward", flush=True)
+    p = net.params().astype(np.float64)
+    fr = net.freqs()
+    rn = r.cpu().numpy()
+    fused = net.predict(r).cpu().numpy()
+    unfused = net.predict(r, unfused=True).cpu().numpy()
+    emu = M.predict(p, fr, CS, rms, rn, bf=True)
+    exact = M.predict(p, fr, CS, rms, rn)
+    print("fused-unfused", _rel(fused, unfused), "fused-bf16 oracle", _rel(fused, emu),
+          "fused-fp64 oracle", _rel(fused, exact))
+    # same BF16 arithmetic: fused and unfused kernels differ only in
+    # accumulation order; the oracle emulates the BF16 operand rounding
+    assert _rel(fused, unfused) < 1e-5
+    assert _rel(fused, emu) < 2e-3
+    assert _rel(fused, exact) < 3e-2
+    net.close()
+    ctx.close()
+
+
+def test_gradient_matches_oracle():
+    from gimic_amd.surrogate import Network
+    s, ctx, r, J, rms = _setup()
+    net = Network(2.0, CS, rms, seed=1)
+    print("case mlp gradient", flush=True)
+    eps = 0.01
+    loss, grad = net.loss_grad(r, J, eps)
+    p = net.params().astype(np.float64)
+    fr = net.freqs()
+    rn, Jn = r.cpu().numpy(), J.cpu().numpy()
+    l_emu, g_emu = M.loss_grad(p, fr, CS, rms, rn, Jn, eps, bf=True)
+    l_64, g_64 = M.loss_grad(p, fr, CS, rms, rn, Jn, eps)
+    print("loss gpu", loss, "bf16 oracle", l_emu, "fp64 oracle", l_64)
+    print("grad rel err vs bf16 oracle", _rel(grad, g_emu), "vs fp64 oracle", _rel(grad, g_64))
+    assert abs(loss - l_emu) <= 1e-3 * l_emu
+    assert _rel(grad, g_emu) < 1e-2
+    assert _rel(grad, g_64) < 5e-2
+    # per layer too, so no layer can hide behind a larger one
+    lay, _ = M.layout()
+    for w, b, kin, nout in lay:
+        assert _rel(grad[w:b], g_emu[w:b]) < 2e-2
+        assert _rel(grad[b:b + nout], g_emu[b:b + nout]) < 2e-2
+    net.close()
+    ctx.close()
+
+
+def test_params_roundtrip_and_step():
+    from gimic_amd.surrogate import Network
+    s, ctx, r, J, rms = _setup(1024)
+    net = Network(2.0, CS, rms, seed=2)
+    p0 = net.params()
+    net.set_params(p0 * 0.5)
+    assert np.array_equal(net.params(), p0 * 0.5)
+    net.set_params(p0)
+    # first Adam step moves every parameter with a nonzero gradient by ~lr
+    _, g = net.loss_grad(r, J, 0.01)
+    net.train_step(r, J, 0.01, 1e-3)
+    d = net.params() - p0
+    # (update lr * g / (|g| + 1e-8): exactly lr up to 1e-3 for |g| > 1e-5)
+    live = np.abs(g) > 1e-5
+    assert np.allclose(np.abs(d[live]), 1e-3, rtol=1e-2)
+    assert np.all(np.sign(d[live]) == -np.sign(g[live]))
+    net.close()
+    ctx.close()
+
+
+def test_training_reduces_loss():
+    import torch
+    from gimic_amd.surrogate import Network
+    s, ctx, r, J, rms = _setup(65536, seed=4)
+    net = Network(1.0, CS, rms, seed=3)
+    print("case mlp training", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(9)
+    l0 = net.loss_grad(r, J, 0.01)[0]
+    for it in range(600):
+        idx = torch.randint(0, r.shape[0], (8192,), generator=g, device="cuda")
+        net.train_step(r[idx], J[idx], 0.01, 2e-3)
+    l1 = net.loss_grad(r, J, 0.01)[0]
+    print("loss", l0, "->", l1)
+    assert l1 < 0.1 * l0
+    net.close()
+    ctx.close()
+
+
+def test_fallback_covers_nuclei_and_unvalidated_cells():
+    import torch
+    from gimic_amd import synth
+    from gimic_amd.surrogate import Surrogate
+    s = synth.ring12()
+    ctx = s.context()
+    coords = np.asarray(s.coords)
+    sg = Surrogate(ctx, coords, -6 * np.ones(3), 6 * np.ones(3), tol=1e-3, ncell=4, seed=1)
+    print("case surrogate fallback", flush=True)
+    st = sg.fit(n_train=60000, steps=200, batch=4096)
+    assert st["n_holdout"] == 600
+    r = torch.tensor(np.r_[coords + 0.1, np.random.default_rng(0).uniform(-6, 6, (4000, 3))],
+                     dtype=torch.float64, device="cuda")
+    Jh, served = sg.evaluate(r)
+    served = served.cpu().numpy()
+    assert not served[:len(coords)].any()            # near nuclei: exact
+    ex = ctx.jtensor(r.cpu().numpy(), precision="mixed")
+    Jh = Jh.cpu().numpy()
+    # fallback is the exact kernel (batch composition changes only the FP64
+    # summation order of the per-tile partial sums)
+    assert np.allclose(Jh[~served], ex[~served], rtol=1e-12, atol=1e-15 * np.abs(ex).max())
+    ok = sg.cell_ok.cpu().numpy()
+    cells = sg.cell_index(r).cpu().numpy()
+    assert np.array_equal(served, ok[cells] & ~sg.near_nucleus(r).cpu().numpy())
+    ctx.close()
