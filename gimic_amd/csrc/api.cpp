@@ -104,19 +104,23 @@ struct gimic_ctx {
     // right operands per (spin list, precision)
     struct Mats {
         DevBuf b64, b32;
+        DevBuf bs3;  // BF16 parts in LDS-image blocks (pre-split mixed kernel)
         int nmatset = 0;
     };
     std::map<std::pair<std::vector<int>, int>, std::unique_ptr<Mats>> mats;
     // workspace
-    DevBuf phi64, grad64, phi32, grad32, part, pts_tmp, jt_tmp, contrib, planes,
+    DevBuf phi64, grad64, phi32, grad32, phis, part, pts_tmp, jt_tmp, contrib, planes,
         pparams, offs, outs;
     // optional per-kernel timing with HIP events on the launch stream:
     // [0] basis evaluation, [1] contraction, [2] finalize
     bool timing = false;
     // mixed-precision contraction kernel chosen by timing (-1: not yet tuned;
-    // 1 register-tiled, 2 LDS-tiled) and the timings that chose it
+    // 1 register-tiled, 2 LDS-tiled, 3 pre-split operands with direct-to-LDS
+    // staging) and the timings (basis evaluation + contraction) that chose it
     int mixed_variant = -1;
-    float tune_ms[2] = {0.f, 0.f};
+    int mixed_split = 0;  // N-split count of the pre-split kernel (0: by workgroup count)
+    float tune_ms[3] = {0.f, 0.f, 0.f};
+    std::vector<std::pair<int, float>> tune_log;  // (variant * 100 + split, ms)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
     double ms[3] = {0.0, 0.0, 0.0};
     long long launches[3] = {0, 0, 0};
@@ -255,6 +259,50 @@ gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec)
         hip_check(hipMemcpyAsync(m->b32.p, b32.data(), b32.size() * sizeof(float),
                                  hipMemcpyHostToDevice, st),
                   "hipMemcpyAsync");
+        // the same FP32 values as three BF16 parts (x = h + m + l exactly),
+        // one block per (set, N-tile, K-step) laid out as the pre-split
+        // kernel's LDS buffer: [part][k-group g][column][8 k], column =
+        // matrix * 32 + function within the N-tile
+        const size_t nks = ld / 32, ntl = nc / 32, blk = kMx2BlockElems;
+        std::vector<uint16_t> bs(spins.size() * ntl * nks * blk, 0);
+        auto bf = [](float x) {
+            uint32_t u;
+            std::memcpy(&u, &x, 4);
+            u += 0x7FFFu + ((u >> 16) & 1u);
+            return (uint16_t)(u >> 16);
+        };
+        auto fl = [](uint16_t h) {
+            const uint32_t u = (uint32_t)h << 16;
+            float x;
+            std::memcpy(&x, &u, 4);
+            return x;
+        };
+        const size_t plane = 4 * (size_t)kMx2Cols * 8;
+        for (size_t si = 0; si < spins.size(); si++)
+            for (size_t nt = 0; nt < ntl; nt++)
+                for (int col = 0; col < kMx2Cols; col++) {
+                    const size_t mat = col / 32, nu = nt * 32 + col % 32;
+                    const float *src = &b32[si * setsz + mat * matsz + nu * ld];
+                    for (size_t ks = 0; ks < nks; ks++) {
+                        uint16_t *dst = &bs[((si * ntl + nt) * nks + ks) * blk];
+                        for (int gk = 0; gk < 4; gk++)
+                            for (int e = 0; e < 8; e++) {
+                                const float x = src[ks * 32 + gk * 8 + e];
+                                const uint16_t h = bf(x);
+                                const float r1 = x - fl(h);
+                                const uint16_t mm = bf(r1);
+                                const uint16_t l = bf(r1 - fl(mm));
+                                const size_t o = ((size_t)gk * kMx2Cols + col) * 8 + e;
+                                dst[o] = h;
+                                dst[plane + o] = mm;
+                                dst[2 * plane + o] = l;
+                            }
+                    }
+                }
+        m->bs3.alloc(bs.size() * 2, st, false);
+        hip_check(hipMemcpyAsync(m->bs3.p, bs.data(), bs.size() * 2, hipMemcpyHostToDevice, st),
+                  "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     }
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     auto &ref = *m;
@@ -323,11 +371,12 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
 
     const size_t ld = (size_t)c->lda;
     // chunk of points per pass: basis values + gradients of one chunk stay
-    // within ~4 GiB (FP64: 32 bytes per point and function)
-    const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 16);
+    // within ~8 GiB (FP64: 32 bytes per point and function)
+    // (mixed: FP32 values 4 + split values 6 + FP32 gradients 12 bytes)
+    const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 22);
     // row granularity: the workgroup height of the contraction kernels (128)
     const size_t gran = prec == GIMIC_PREC_FP64 ? (size_t)kWgRows : (size_t)contract_mx_rows();
-    size_t cap = (size_t)4 << 30;
+    size_t cap = (size_t)8 << 30;
     size_t chunk = std::max<size_t>(gran, (cap / per_row) / gran * gran);
     chunk = std::min(chunk, round_up((size_t)npts, gran));
     const size_t rows_cap = chunk;
@@ -339,6 +388,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         grow(c->grad64, 3 * rows_cap * ld * 8, c->grad64.bytes < 3 * rows_cap * ld * 8);
     } else {
         grow(c->phi32, rows_cap * ld * 4, c->phi32.bytes < rows_cap * ld * 4);
+        grow(c->phis, rows_cap * ld * 6, c->phis.bytes < rows_cap * ld * 6);
         grow(c->grad32, 3 * rows_cap * ld * 4, c->grad32.bytes < 3 * rows_cap * ld * 4);
     }
     const int nmatset = M.nmatset;
@@ -363,6 +413,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         ea.grad64 = c->grad64.as<double>();
         ea.phi32 = c->phi32.as<float>();
         ea.grad32 = c->grad32.as<float>();
+        ea.phis = c->phis.as<__bf16>();
 
         ContractArgs ca{};
         ca.rows = rows;
@@ -378,28 +429,40 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         ca.a32 = c->phi32.as<float>();
         ca.g32 = c->grad32.as<float>();
         ca.b32 = M.b32.as<float>();
+        ca.as3 = c->phis.as<__bf16>();
+        ca.bs3 = M.bs3.as<__bf16>();
         ca.rf64 = c->rf64.as<double>();
         ca.rf32 = c->rf32.as<float>();
-        // work split over N so a launch has at least ~2048 workgroups
-        auto plan = [&](int wg_rows, int tile_cols) {
+        // work split over N so a launch has at least ~2048 workgroups, or
+        // the split count the tuner chose for the pre-split kernel
+        auto plan = [&](int wg_rows, int tile_cols, int want_split = 0) {
             ca.ntiles = c->ncol_pad / tile_cols;
             const long long per = (long long)(rows / wg_rows) * nmatset;
             int nsplit = (int)std::min<long long>(ca.ntiles,
                                                   std::max<long long>(1, (2048 + per - 1) / per));
+            if (want_split > 0) nsplit = std::min(want_split, ca.ntiles);
             ca.tiles_per_split = (ca.ntiles + nsplit - 1) / nsplit;
             ca.nsplit = (ca.ntiles + ca.tiles_per_split - 1) / ca.tiles_per_split;
             grow(c->part, (size_t)nmatset * ca.nsplit * kNQ * rows * 8, false);
             ca.part = c->part.as<double>();
         };
-        auto plan_variant = [&](int variant) {
-            if (variant == 2) plan(contract_mx_rows(), contract_mx_cols());
+        auto plan_variant = [&](int variant, int split = -1) {
+            if (split < 0) split = c->mixed_split;
+            if (variant == 3) plan(contract_mx_rows(), contract_mx_cols(), split);
+            else if (variant == 2) plan(contract_mx_rows(), contract_mx_cols());
             else plan(kWgRows, kNT);
         };
-        auto run = [&](int variant) {
-            plan_variant(variant);
+        auto run = [&](int variant, int split = -1) {
+            plan_variant(variant, split);
             if (variant == 0) hip_check(launch_contract_f64(ca, st), "contract_f64");
             else if (variant == 1) hip_check(launch_contract_bf16x6(ca, st), "contract_bf16x6");
-            else hip_check(launch_contract_mx(ca, st), "contract_mx");
+            else if (variant == 2) hip_check(launch_contract_mx(ca, st), "contract_mx");
+            else hip_check(launch_contract_mx2(ca, st), "contract_mx2");
+        };
+        // basis evaluation output each contraction kernel reads
+        auto eval_mode = [](int variant) { return variant == 0 ? 0 : variant == 3 ? 2 : 1; };
+        auto eval = [&](int variant) {
+            hip_check(launch_basis_eval(ea, eval_mode(variant), st), "basis_eval");
         };
         const bool tune = prec == GIMIC_PREC_MIXED && c->mixed_variant < 0;
         if (capture && tune) throw Error(GIMIC_ESTATE, "mixed kernel not tuned before capture");
@@ -407,37 +470,47 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             plan_variant(prec == GIMIC_PREC_FP64 ? 0 : c->mixed_variant);
             continue;
         }
-        timed(c, 0, st, [&] {
-            hip_check(launch_basis_eval(ea, prec == GIMIC_PREC_FP64 ? 0 : 1, st), "basis_eval");
-        });
         if (tune) {
-            // autotune once per context: time both mixed kernels on this chunk
+            // autotune once per context on this chunk: basis evaluation +
+            // contraction for every mixed kernel, and for the pre-split
+            // kernel every N-split count (1 = one pass over all columns;
+            // more splits give more workgroups that share each row tile's A
+            // through L2); keep the fastest
             float best = 0.f;
-            for (int v = 1; v <= 2; v++) {
+            int best_split = 0;
+            const int cand[][2] = {{1, 0}, {2, 0}, {3, 2}, {3, 4}, {3, 8}, {3, 16}};
+            c->tune_log.clear();
+            for (auto &cv : cand) {
+                const int v = cv[0], sp = cv[1];
                 hipEvent_t e0, e1;
                 hip_check(hipEventCreate(&e0), "hipEventCreate");
                 hip_check(hipEventCreate(&e1), "hipEventCreate");
-                plan_variant(v);  // allocate before the timed region
+                plan_variant(v, sp);  // allocate before the timed region
                 hip_check(hipEventRecord(e0, st), "hipEventRecord");
-                run(v);
+                eval(v);
+                run(v, sp);
                 hip_check(hipEventRecord(e1, st), "hipEventRecord");
                 hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
                 float ms = 0.f;
                 hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
                 (void)hipEventDestroy(e0);
                 (void)hipEventDestroy(e1);
-                c->tune_ms[v - 1] = ms;
+                c->tune_log.push_back({v * 100 + sp, ms});
+                if (c->tune_ms[v - 1] == 0.f || ms < c->tune_ms[v - 1]) c->tune_ms[v - 1] = ms;
                 if (c->mixed_variant < 0 || ms < best) {
                     best = ms;
                     c->mixed_variant = v;
+                    best_split = sp;
                 }
             }
+            c->mixed_split = best_split;
             if (mode == EvalMode::Reserve) {
                 plan_variant(c->mixed_variant);
                 continue;
             }
         }
         const int variant = prec == GIMIC_PREC_FP64 ? 0 : c->mixed_variant;
+        timed(c, 0, st, [&] { eval(variant); });
         timed(c, 1, st, [&] { run(variant); });
         const int nsplit = ca.nsplit;
 
@@ -597,12 +670,12 @@ int gimic_ctx_get_timing(gimic_ctx *c, double *ms, long long *launches) {
     });
 }
 
-int gimic_ctx_tuning(const gimic_ctx *c, int *variant, double *ms) {
+int gimic_ctx_tuning(const gimic_ctx *c, int *variant, int *split, double *ms) {
     return guarded([&] {
-        if (!c || !variant || !ms) throw Error(GIMIC_EINVAL, "null argument");
+        if (!c || !variant || !split || !ms) throw Error(GIMIC_EINVAL, "null argument");
         *variant = c->mixed_variant;
-        ms[0] = c->tune_ms[0];
-        ms[1] = c->tune_ms[1];
+        *split = c->mixed_split;
+        for (int k = 0; k < 3; k++) ms[k] = c->tune_ms[k];
     });
 }
 
@@ -762,6 +835,17 @@ int gimic_flux_planes(gimic_ctx *c, int nplanes, const int64_t *offs, const doub
         if (out_mod) std::copy(res.begin() + 3 * nplanes, res.begin() + 6 * nplanes, out_mod);
         if (out_acid)
             for (int k = 0; k < nplanes; k++) out_acid[k] = res[6 * (size_t)nplanes + 3 * k];
+    });
+}
+
+int gimic_ctx_set_mixed_kernel(gimic_ctx *c, int variant, int split) {
+    return guarded([&] {
+        if (!c) throw Error(GIMIC_EINVAL, "null context");
+        if (variant < 0 || variant > 3) throw Error(GIMIC_EINVAL, "mixed kernel must be 0..3");
+        if (split < 0) throw Error(GIMIC_EINVAL, "split count must be >= 0");
+        c->mixed_variant = variant == 0 ? -1 : variant;
+        c->mixed_split = variant == 3 ? split : 0;
+        for (auto &e : c->tune_ms) e = 0.f;
     });
 }
 

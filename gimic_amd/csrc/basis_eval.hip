@@ -76,8 +76,22 @@ __device__ __forceinline__ void store(const EvalArgs &a, size_t row, int f, doub
         a.grad64[i] = gx;
         a.grad64[plane + i] = gy;
         a.grad64[2 * plane + i] = gz;
-    } else {
+    } else if (MODE == 1) {
         a.phi32[i] = (float)v;
+        a.grad32[i] = (float)gx;
+        a.grad32[plane + i] = (float)gy;
+        a.grad32[2 * plane + i] = (float)gz;
+    } else {
+        // value as three BF16 parts, x = h + m + l exactly
+        const float x = (float)v;
+        const __bf16 hi = (__bf16)x;
+        const float r1 = x - (float)hi;
+        const __bf16 mi = (__bf16)r1;
+        const __bf16 lo = (__bf16)(r1 - (float)mi);
+        __bf16 *d = a.phis + row * 3 * (size_t)a.lda + (f >> 3) * 24 + (f & 7);
+        d[0] = hi;
+        d[8] = mi;
+        d[16] = lo;
         a.grad32[i] = (float)gx;
         a.grad32[plane + i] = (float)gy;
         a.grad32[2 * plane + i] = (float)gz;
@@ -199,8 +213,10 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
     if (total <= 0) return hipSuccess;
     if (mode == 0)
         hipLaunchKernelGGL(basis_eval_kernel<0>, dim3(nblk), dim3(256), 0, s, a);
-    else
+    else if (mode == 1)
         hipLaunchKernelGGL(basis_eval_kernel<1>, dim3(nblk), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(basis_eval_kernel<2>, dim3(nblk), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -1,0 +1,242 @@
+// Mixed-precision contraction with pre-split operands (gfx950).
+//
+// Arithmetic as in contract_bf16x6_kernel: each FP32 operand value is the
+// exact sum of three BF16 parts h + m + l, a block product keeps the six
+// terms hh, hm, mh, hl, mm, lh (smallest first) on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation, and FP64 partial sums
+// are kept across N-tiles.  What differs is where the splitting happens:
+//   * the basis kernel writes the left operand already split, [row][k/8][3][8]
+//     (48 contiguous bytes per lane and K-step), and
+//   * the right operand (constant per density) is split once on the host into
+//     44 KiB blocks per (density set, N-tile, K-step) that are byte images of
+//     the kernel's LDS buffer, [plane][k-group][column][8].
+// The K-loop therefore has no conversion work: each K-step streams the next
+// block into the other LDS buffer with gfx950's direct global-to-LDS loads
+// (global_load_lds_dwordx4: no VGPRs, no ds_write), loads the next A
+// fragments, and runs 168 MFMAs per wavefront against B fragments that are
+// read from LDS one column block ahead.
+//
+// Tile: 128 point rows per workgroup (4 wavefronts x 32 rows), 224 columns
+// per N-tile (7 matrices x 32 functions), 2 x 14 MFMA blocks per wavefront.
+//
+// Workgroup order.  Every workgroup re-reads its 128 rows of A once per
+// N-tile, so A traffic is what limits this kernel.  The grid is 1-D and
+// decoded so that the nsplit workgroups that share a row tile (one per
+// N-range) are dispatched back to back on the same XCD: the hardware hands
+// workgroup i to XCD i mod 8, so slot s = i / 8 of XCD x runs row tile
+// 8 * (s / nsplit) + x with split s % nsplit.  They stream the same A rows
+// at the same time and all but one read them from that XCD's L2.
+#include "device_common.h"
+
+namespace gimic {
+
+namespace {
+
+constexpr int kRows = 128;
+constexpr int kNu = 32;
+constexpr int kCols = kMx2Cols;             // 224
+constexpr int kCB = kCols / 16;             // 14 column blocks
+constexpr int kRB = 2;                      // row blocks per wavefront
+constexpr int kBlk = kMx2BlockElems;        // BF16 per block / LDS buffer
+constexpr int kPlane = 4 * kCols * 8;       // BF16 per part plane of a block
+constexpr int kPieces = kBlk * 2 / 1024;    // 1 KiB pieces per block (44)
+constexpr int kPiecesPerWave = kPieces / 4; // 11
+constexpr size_t kLdsB = 2 * (size_t)kBlk * 2;
+constexpr size_t kLdsQ = (size_t)kRows * kNQ * 8;
+static_assert(kPieces % 4 == 0, "block must split evenly over 4 wavefronts");
+static_assert(3 * kPlane <= kBlk, "block too small");
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// one 44 KiB block into an LDS buffer: wavefront w copies pieces w, w+4, ...
+// (64 lanes x 16 bytes per instruction, LDS base in M0)
+__device__ __forceinline__ void stage_block(const __bf16 *blk, unsigned char *buf, int wave,
+                                            int lane) {
+#pragma unroll
+    for (int i = 0; i < kPiecesPerWave; i++) {
+        const int piece = wave + 4 * i;
+        const unsigned char *src = (const unsigned char *)blk + piece * 1024 + lane * 16;
+        __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(buf + piece * 1024), 16,
+                                         0, 0);
+    }
+}
+
+__device__ __forceinline__ v4f mfma6(const v8bf (&a)[3], const v8bf (&b)[3], v4f acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
+    return acc;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned char *bbuf = smem;
+    double *qs = (double *)(smem + kLdsB);
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = lane >> 4, c = lane & 15;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int groups = (a.rows / kRows + 7) >> 3;  // groups of 8 row tiles
+    const int per_set = groups * a.nsplit;
+    const int set = slot / per_set;
+    const int split = slot % a.nsplit;
+    const int rtile = ((slot % per_set) / a.nsplit) * 8 + xcd;
+    if (rtile >= a.rows / kRows) return;  // padding of the last group
+    const size_t row0 = (size_t)rtile * kRows;
+    const int lda = a.lda;
+    const int nks = lda / 32;
+    const size_t rows = (size_t)a.rows;
+    const size_t gplane = (size_t)a.gplane;
+    const size_t arow = 3 * (size_t)lda;  // BF16 per split row
+    for (int i = tid; i < kRows * kNQ; i += 256) qs[i] = 0.0;
+
+    // lane (g, c) of row block rb: row wave*32 + rb*16 + c, k = 32 ks + 8 g .. + 7
+    const __bf16 *Aw = a.as3 + (row0 + wave * 32 + c) * arow + g * 24;
+    const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kBlk;
+    const int nt0 = split * a.tiles_per_split;
+    const int nt1 = min(nt0 + a.tiles_per_split, a.ntiles);
+
+    for (int nt = nt0; nt < nt1; nt++) {
+        const __bf16 *Bnt = Bset + (size_t)nt * nks * kBlk;
+        v4f acc[kRB][kCB];
+#pragma unroll
+        for (int rb = 0; rb < kRB; rb++)
+#pragma unroll
+            for (int cb = 0; cb < kCB; cb++) acc[rb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
+
+        stage_block(Bnt, bbuf, wave, lane);
+        v8bf an[kRB][3];
+#pragma unroll
+        for (int rb = 0; rb < kRB; rb++)
+#pragma unroll
+            for (int p = 0; p < 3; p++) an[rb][p] = *(const v8bf *)(Aw + rb * 16 * arow + p * 8);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+
+        for (int ks = 0; ks < nks; ks++) {
+            v8bf as[kRB][3];
+#pragma unroll
+            for (int rb = 0; rb < kRB; rb++)
+#pragma unroll
+                for (int p = 0; p < 3; p++) as[rb][p] = an[rb][p];
+            if (ks + 1 < nks) {
+                stage_block(Bnt + (size_t)(ks + 1) * kBlk, bbuf + (size_t)((ks + 1) & 1) * kBlk * 2,
+                            wave, lane);
+#pragma unroll
+                for (int rb = 0; rb < kRB; rb++)
+#pragma unroll
+                    for (int p = 0; p < 3; p++)
+                        an[rb][p] = *(const v8bf *)(Aw + rb * 16 * arow + (size_t)(ks + 1) * 96 + p * 8);
+            }
+            const __bf16 *cur = (const __bf16 *)(bbuf + (size_t)(ks & 1) * kBlk * 2);
+            v8bf bf[2][3];
+            {
+                const int off = (g * kCols + c) * 8;
+#pragma unroll
+                for (int p = 0; p < 3; p++) bf[0][p] = *(const v8bf *)(cur + p * kPlane + off);
+            }
+#pragma unroll
+            for (int cb = 0; cb < kCB; cb++) {
+                const int sl = cb & 1;
+                if (cb + 1 < kCB) {
+                    const int off = (g * kCols + (cb + 1) * 16 + c) * 8;
+#pragma unroll
+                    for (int p = 0; p < 3; p++)
+                        bf[sl ^ 1][p] = *(const v8bf *)(cur + p * kPlane + off);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int rb = 0; rb < kRB; rb++) acc[rb][cb] = mfma6(as[rb], bf[sl], acc[rb][cb]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            // this wavefront's block copy and A loads have landed; after the
+            // barrier every wavefront's copy is visible and nobody reads the
+            // buffer that the next K-step overwrites
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
+
+        // epilogue: lane (g, c) holds Y[row 4g + r][col c] of every block;
+        // column block cb = 2 * matrix + half covers functions nt*32 + 16*half + c
+        float R[2][3];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) R[h][d] = a.rf32[(size_t)d * a.ncol_pad + nt * kNu + 16 * h + c];
+#pragma unroll
+        for (int rb = 0; rb < kRB; rb++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rl = wave * 32 + rb * 16 + 4 * g + r;
+                const size_t p = row0 + rl;
+                const bool valid = p < (size_t)a.npts;
+                const float rp[3] = {valid ? (float)a.pts[3 * p] : 0.f,
+                                     valid ? (float)a.pts[3 * p + 1] : 0.f,
+                                     valid ? (float)a.pts[3 * p + 2] : 0.f};
+                float q[kNQ];
+#pragma unroll
+                for (int i = 0; i < kNQ; i++) q[i] = 0.f;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int col = nt * kNu + 16 * h + c;
+                    const __bf16 *pe = a.as3 + p * arow + (col >> 3) * 24 + (col & 7);
+                    const float ph = ((float)pe[0] + (float)pe[8]) + (float)pe[16];
+                    const size_t e = p * lda + col;
+                    const float G[3] = {a.g32[e], a.g32[gplane + e], a.g32[2 * gplane + e]};
+                    const float t0 = acc[rb][h][r] * ph;
+                    q[0] += t0;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) q[1 + d] += t0 * (rp[d] - R[h][d]);
+#pragma unroll
+                    for (int bb = 0; bb < 3; bb++)
+#pragma unroll
+                        for (int m = 0; m < 3; m++) q[4 + m + 3 * bb] += acc[rb][2 * (1 + bb) + h][r] * G[m];
+#pragma unroll
+                    for (int d = 0; d < 3; d++)
+#pragma unroll
+                        for (int m = 0; m < 3; m++) q[13 + m + 3 * d] += G[m] * acc[rb][2 * (4 + d) + h][r];
+                }
+#pragma unroll
+                for (int off = 1; off < 16; off <<= 1)
+#pragma unroll
+                    for (int i = 0; i < kNQ; i++) q[i] += __shfl_xor(q[i], off);
+#pragma unroll
+                for (int i = 0; i < kNQ; i++)
+                    if ((i & 15) == c) qs[rl * kNQ + i] += (double)q[i];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+    __syncthreads();
+    double *out = a.part + ((size_t)set * a.nsplit + split) * kNQ * rows + row0;
+    for (int i = tid; i < kRows * kNQ; i += 256) {
+        const int q = i / kRows, rl = i % kRows;
+        out[(size_t)q * rows + rl] = qs[rl * kNQ + q];
+    }
+}
+
+hipError_t launch_contract_mx2(const ContractArgs &a, hipStream_t s) {
+    static bool attr = false;
+    const size_t lds = kLdsB + kLdsQ;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)contract_mx2_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    if (a.rows % kRows || a.lda % 32 || a.ncol_pad != a.lda || a.ntiles * kNu != a.ncol_pad)
+        return hipErrorInvalidValue;
+    const long long groups = (a.rows / kRows + 7) / 8;
+    const long long nwg = 8 * groups * a.nsplit * a.nmatset;
+    if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(contract_mx2_kernel, dim3((unsigned)nwg), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace gimic

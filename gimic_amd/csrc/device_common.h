@@ -65,6 +65,9 @@ struct EvalArgs {
     double *phi64, *grad64;  // [rows][lda], [3][rows][lda]
     // mixed outputs (FP32 values; the contraction splits them into BF16)
     float *phi32, *grad32;   // [rows][lda], [3][rows][lda]
+    // split basis values for the pre-split contraction: each FP32 value x as
+    // three BF16 parts x = h + m + l (exact), layout [row][lda/8][3][8]
+    __bf16 *phis;
 };
 hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s);
 hipError_t upload_basis_tables(const double *sph, int nsph, hipStream_t s);
@@ -78,6 +81,8 @@ struct ContractArgs {
     int npts;
     const double *a64, *g64, *b64;  // FP64 path
     const float *a32, *g32, *b32;   // mixed path (FP32 operands, split to BF16 x3)
+    const __bf16 *as3;   // split basis values [rows][lda/8][3][8] (pre-split kernel)
+    const __bf16 *bs3;   // split right operand, LDS-image blocks per (set, N-tile, K-step)
     const double *rf64;  // [3][ncol_pad] atom coordinate of each function
     const float *rf32;
     double *part;        // [nmatset][nsplit][kNQ][rows]
@@ -87,6 +92,11 @@ hipError_t launch_contract_bf16x6(const ContractArgs &a, hipStream_t s);
 hipError_t launch_contract_mx(const ContractArgs &a, hipStream_t s);
 int contract_mx_rows();  // row tile of launch_contract_mx (a multiple of kWgRows)
 int contract_mx_cols();  // functions per N-tile of launch_contract_mx
+// Pre-split variant: both operands arrive as BF16 parts, the right operand in
+// blocks that are byte images of the kernel's LDS buffer (see contract_mx2.hip)
+hipError_t launch_contract_mx2(const ContractArgs &a, hipStream_t s);
+constexpr int kMx2Cols = 224;                         // 7 matrices x 32 functions
+constexpr int kMx2BlockElems = 22528;                 // 3*4*224*8 = 21504, padded to 44 KiB
 
 struct FinalizeArgs {
     const double *part;  // [nmatset][nsplit][kNQ][rows]

@@ -3,11 +3,13 @@
 // Network (per molecule): u = (r - centre) / scale;
 //   f = [sin(2 pi B u), cos(2 pi B u), u, 0...]   (2F + 3 features, padded to K0)
 //   h1..hD = SiLU(W_l h + b_l)                      (width H)
-//   y = W_out hD + b_out                            (9 outputs, padded to 16)
-//   J = y * out_scale
+//   y = W_out hD + b_out                            (16 rows: 9 used + 1 log scale)
+//   J_i = out_scale_i * y_i * exp(y_9)              (i < 9)
+// The learned log amplitude y_9 lets the network follow a field whose size
+// changes by orders of magnitude while y_0..8 stay of order one.
 // Training: BF16 operands on MFMA (v_mfma_f32_16x16x32_bf16) with FP32
 // accumulation, FP32 master weights and Adam; loss = mean over points of
-// sum_i (y_i - t_i)^2 / (|t|^2 + eps^2) with t = J / out_scale.
+// sum_i (y_i e^{y_9} - t_i)^2 / (|t|^2 + eps^2) with t = J / out_scale.
 // Inference: one fused kernel per 64-point tile, activations in LDS.
 //
 // The generic GEMM below takes arbitrary element strides for both operands
@@ -28,6 +30,8 @@ constexpr int kGT = 64;   // GEMM tile (M and N)
 constexpr int kGK = 32;   // K per step
 
 __device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+// exp of the log amplitude, bounded so that no product overflows FP32
+__device__ __forceinline__ float amp(float z) { return __expf(fminf(z, 60.f)); }
 __device__ __forceinline__ float dsilu(float z) {
     const float s = 1.f / (1.f + __expf(-z));
     return s * (1.f + z * (1.f - s));
@@ -127,7 +131,9 @@ __global__ void mlp_bias_silu_kernel(float *Z, const float *b, __bf16 *H, long l
     H[i] = (__bf16)silu(z);
 }
 
-// output layer: Y = Z + b; loss terms and dY (fp32 and bf16) for the 9 outputs
+// output layer: Y = Z + b; loss terms and dY (fp32 and bf16): with
+// q_i = y_i e^{y_9} and d_i = q_i - t_i, dL/dy_i = s d_i e^{y_9} and
+// dL/dy_9 = s sum_i d_i q_i, s = 2 w / npts
 __global__ void mlp_loss_kernel(const float *Z, const float *b, const double *target,
                                 const float *out_scale, float eps2, long long npts, int NO,
                                 float *dY, __bf16 *dYb, float *loss_acc) {
@@ -142,16 +148,24 @@ __global__ void mlp_loss_kernel(const float *Z, const float *b, const double *ta
             y[i] = Z[p * NO + i] + b[i];
             tn += t[i] * t[i];
         }
+        const float z9 = Z[p * NO + 9] + b[9];
+        const float e = amp(z9);
         const float w = 1.f / (tn + eps2);
         const float s = 2.f * w / (float)npts;
+        float dz = 0.f;
 #pragma unroll
         for (int i = 0; i < 9; i++) {
-            const float d = y[i] - t[i];
+            const float q = y[i] * e;
+            const float d = q - t[i];
             l += w * d * d;
-            dY[p * NO + i] = s * d;
-            dYb[p * NO + i] = (__bf16)(s * d);
+            dz += d * q;
+            dY[p * NO + i] = s * d * e;
+            dYb[p * NO + i] = (__bf16)(s * d * e);
         }
-        for (int i = 9; i < NO; i++) {
+        dz = z9 < 60.f ? s * dz : 0.f;  // the amplitude is capped at e^60
+        dY[p * NO + 9] = dz;
+        dYb[p * NO + 9] = (__bf16)dz;
+        for (int i = 10; i < NO; i++) {
             dY[p * NO + i] = 0.f;
             dYb[p * NO + i] = (__bf16)0.f;
         }
@@ -213,13 +227,14 @@ __global__ void mlp_cast_kernel(const float *p, __bf16 *pb, long long n) {
     if (i < n) pb[i] = (__bf16)p[i];
 }
 
-// final output: J[p][i] = (Z[p][i] + b[i]) * out_scale[i] (fp64)
+// final output: J[p][i] = (Z[p][i] + b[i]) exp(Z[p][9] + b[9]) out_scale[i] (fp64)
 __global__ void mlp_output_kernel(const float *Z, const float *b, const float *out_scale,
                                   long long npts, int NO, double *J) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npts) return;
+    const float e = amp(Z[p * NO + 9] + b[9]);
 #pragma unroll
-    for (int i = 0; i < 9; i++) J[9 * p + i] = (double)((Z[p * NO + i] + b[i]) * out_scale[i]);
+    for (int i = 0; i < 9; i++) J[9 * p + i] = (double)((Z[p * NO + i] + b[i]) * e * out_scale[i]);
 }
 
 // ---- fused inference: one 64-point tile per workgroup ----
@@ -291,7 +306,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const double *r, long lo
         __syncthreads();
         cur ^= 1;
     }
-    // output layer (16 columns, 9 used)
+    // output layer (16 columns: 9 used + the log amplitude in column 9)
     {
         const __bf16 *Wo = W[D];
         const float *bo = bias[D];
@@ -304,7 +319,10 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const double *r, long lo
 #pragma unroll
         for (int rr = 0; rr < 4; rr++) {
             const long long p = p0 + wave * 16 + 4 * g + rr;
-            if (c < 9 && p < npts) J[9 * p + c] = (double)((acc[rr] + bo[c]) * out_scale[c]);
+            const float y = acc[rr] + bo[c];
+            // lane (g, 9) holds the log amplitude of the same point
+            const float e = amp(__shfl(y, (lane & ~15) | 9));
+            if (c < 9 && p < npts) J[9 * p + c] = (double)(y * e * out_scale[c]);
         }
     }
 }

@@ -19,7 +19,7 @@ EXPORTS = [
     "gimic_set_screening", "gimic_calc_jtensor", "gimic_calc_jvector", "gimic_calc_modj",
     "gimic_get_gauss_points", "mkgausspoints", "gimic_last_error", "gimic_ctx_create_files",
     "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
-    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
+    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
     "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_sph_transform", "gimic_quadrature",
     "gimic_selftest_mfma_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_set_stream", "gimic_mlp_info",
     "gimic_mlp_params", "gimic_mlp_loss_grad", "gimic_mlp_train_step", "gimic_mlp_predict",
@@ -74,7 +74,8 @@ def load():
         "gimic_ctx_atoms": [P, P, P],
         "gimic_ctx_set_timing": [P, I],
         "gimic_ctx_get_timing": [P, P, P],
-        "gimic_ctx_tuning": [P, P, P],
+        "gimic_ctx_tuning": [P, P, P, P],
+        "gimic_ctx_set_mixed_kernel": [P, I, I],
         "gimic_ctx_set_graph": [P, I],
         "gimic_ctx_graph_stats": [P, P, P],
         "gimic_jtensor_batch": [P, I64, P, P, I, I],
@@ -212,11 +213,18 @@ class Context:
                 for i, k in enumerate(["basis_eval", "contract", "finalize"])}
 
     def tuning(self):
-        """(chosen mixed kernel variant, [ms variant 1, ms variant 2])"""
-        v = ctypes.c_int()
-        ms = np.zeros(2)
-        check(self._lib.gimic_ctx_tuning(self._h, ctypes.byref(v), ptr(ms)))
-        return v.value, ms.tolist()
+        """(chosen mixed kernel variant, its N-split count, [best ms of
+        variants 1, 2, 3])"""
+        v, sp = ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(3)
+        check(self._lib.gimic_ctx_tuning(self._h, ctypes.byref(v), ctypes.byref(sp), ptr(ms)))
+        return v.value, sp.value, ms.tolist()
+
+    def set_mixed_kernel(self, variant=0, split=0):
+        """fix the mixed contraction kernel (1 register-tiled, 2 LDS-tiled,
+        3 pre-split with `split` N-ranges, 0 = by workgroup count) or 0 for
+        automatic selection"""
+        check(self._lib.gimic_ctx_set_mixed_kernel(self._h, int(variant), int(split)))
 
     def set_graph(self, enable=True):
         """capture flux_planes batches into one reusable hipGraph launch"""

@@ -125,10 +125,17 @@ int gimic_ctx_set_timing(gimic_ctx *ctx, int on);
 int gimic_ctx_get_timing(gimic_ctx *ctx, double *ms, long long *launches);
 
 /* Mixed-precision kernel selection (no counterpart in the reference): the
- * first mixed evaluation of a context times the register-tiled (1) and the
- * LDS-tiled (2) contraction kernels on its first chunk and keeps the faster
- * one.  Returns the choice (-1 before the first mixed call) and both times. */
-int gimic_ctx_tuning(const gimic_ctx *ctx, int *variant, double *ms);
+ * first mixed evaluation of a context times basis evaluation + contraction
+ * on its first chunk for the register-tiled (1), the LDS-tiled (2) and the
+ * pre-split (3) kernels, the last with N-split counts 2, 4, 8 and 16, and
+ * keeps the fastest.  Returns the choice (variant -1 before the first mixed
+ * call), its split count (0: chosen by workgroup count) and the best time
+ * in ms of each variant (ms[3]). */
+int gimic_ctx_tuning(const gimic_ctx *ctx, int *variant, int *split, double *ms);
+/* Fix the mixed-precision kernel (1, 2 or 3; for 3 also its N-split count,
+ * 0 = by workgroup count) or return to automatic selection (variant 0: the
+ * next mixed evaluation tunes again). */
+int gimic_ctx_set_mixed_kernel(gimic_ctx *ctx, int variant, int split);
 
 /* Graph mode for gimic_flux_planes (no counterpart in the reference): when
  * enabled, the whole plane batch (basis evaluation, contraction and tensor

@@ -9,9 +9,9 @@ l: W_l[n_out][k_in] row-major, then b_l[n_out]):
     u  = (r - centre) / scale
     x  = [sin(2 pi B u) (64), cos(2 pi B u) (64), u (3), 0 (29)]   k_in 160
     h_l = SiLU(W_l h_{l-1} + b_l), l = 0..3                          width 128
-    y  = W_4 h_3 + b_4                                   16 rows, 9 used
-    J  = y[:9] * out_scale
-Loss: mean_p sum_i (y_i - t_i)^2 / (|t|^2 + eps^2), t = J_target / out_scale.
+    y  = W_4 h_3 + b_4                     16 rows: 9 used + log amplitude y_9
+    J  = y[:9] * exp(min(y_9, 60)) * out_scale
+Loss: mean_p sum_i (y_i e^{y_9} - t_i)^2 / (|t|^2 + eps^2), t = J_target / out_scale.
 
 `bf16=True` rounds every GEMM operand to bfloat16 (round to nearest even)
 where the device kernels do: features, weights, hidden activations, dY and
@@ -86,9 +86,13 @@ def forward(params, freqs, center_scale, r, bf=False):
     return zs[-1], (xs, zs)
 
 
+def amp(y):
+    return np.exp(np.minimum(y[:, NOUT], 60.0))
+
+
 def predict(params, freqs, center_scale, out_scale, r, bf=False):
     y, _ = forward(params, freqs, center_scale, r, bf)
-    return y[:, :NOUT] * np.asarray(out_scale, dtype=np.float64)
+    return y[:, :NOUT] * amp(y)[:, None] * np.asarray(out_scale, dtype=np.float64)
 
 
 def loss_grad(params, freqs, center_scale, out_scale, r, target, eps, bf=False):
@@ -99,10 +103,14 @@ def loss_grad(params, freqs, center_scale, out_scale, r, target, eps, bf=False):
     P = len(y)
     t = np.asarray(target, dtype=np.float64).reshape(P, 9) / np.asarray(out_scale, dtype=np.float64)
     w = 1.0 / ((t * t).sum(axis=1) + eps * eps)
-    d = y[:, :NOUT] - t
+    e = amp(y)
+    q = y[:, :NOUT] * e[:, None]
+    d = q - t
     loss = float((w * (d * d).sum(axis=1)).mean())
+    s = 2.0 * w / P
     dy = np.zeros((P, NO))
-    dy[:, :NOUT] = 2.0 * w[:, None] * d / P
+    dy[:, :NOUT] = s[:, None] * d * e[:, None]
+    dy[:, NOUT] = s * (d * q).sum(axis=1) * (y[:, NOUT] < 60.0)
     lay, n = layout()
     grad = np.zeros(n)
     g = dy

@@ -50,9 +50,12 @@ def test_loss_direct_sum():
     eps = 0.05
     loss, _ = M.loss_grad(p, freqs, cs, os_, r, t, eps)
     J = M.predict(p, freqs, cs, os_, r)
+    y, _ = M.forward(p, freqs, cs, r)
     direct = 0.0
     for i in range(17):
         tt = t[i] / os_
+        Ji = y[i, :9] * np.exp(y[i, 9]) * os_
+        assert np.allclose(Ji, J[i], rtol=1e-14)
         direct += (((J[i] - t[i]) / os_) ** 2).sum() / ((tt ** 2).sum() + eps * eps)
     assert abs(loss - direct / 17) < 1e-12 * abs(loss)
 
@@ -67,7 +70,10 @@ def test_gradient_finite_differences():
     # a few entries of every weight matrix and bias vector
     idx = []
     for w, b, kin, nout in lay:
-        idx += list(rng.integers(w, b, 6)) + list(rng.integers(b, b + min(nout, 9), 3))
+        idx += list(rng.integers(w, b, 6)) + list(rng.integers(b, b + min(nout, 10), 3))
+    # the log-amplitude row of the output layer
+    w4, b4, kin4, _ = lay[-1]
+    idx += list(rng.integers(w4 + 9 * kin4, w4 + 10 * kin4, 4)) + [b4 + 9]
     h = 1e-5
     for i in idx:
         pp, pm = p.copy(), p.copy()
@@ -82,7 +88,9 @@ def test_unused_outputs_have_zero_gradient():
     p, freqs, cs, os_, rng = _net(seed=5)
     _, g = M.loss_grad(p, freqs, cs, os_, rng.normal(0, 3, (5, 3)), rng.normal(0, 1, (5, 9)), 0.1)
     w, b, kin, nout = M.layout()[0][-1]
-    assert np.all(g[w + 9 * kin:b] == 0) and np.all(g[b + 9:b + nout] == 0)
+    # rows 0..8 tensor elements, row 9 the log amplitude, rows 10..15 unused
+    assert np.all(g[w + 10 * kin:b] == 0) and np.all(g[b + 10:b + nout] == 0)
+    assert np.any(g[w + 9 * kin:w + 10 * kin] != 0) and g[b + 9] != 0
 
 
 def test_bf16_mode_close_to_fp64():

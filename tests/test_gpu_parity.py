@@ -151,6 +151,46 @@ def test_mixed_precision_within_stated_tolerance(ring12, c4h4):
     assert e.max() <= MIXED_TOL
 
 
+@pytest.mark.parametrize("variant,split", [(1, 0), (2, 0), (3, 0), (3, 1), (3, 2), (3, 3)])
+def test_each_mixed_kernel_within_tolerance(c4h4, variant, split):
+    """every mixed contraction kernel (register-tiled, LDS-tiled, pre-split
+    with several N-split counts; 1000 points = 8 row tiles, 1300 = 11, so
+    the pre-split kernel's groups of 8 row tiles are full and padded) on
+    ragged points: per-point error vs FP64 within the stated bound"""
+    ctx, b, dens = c4h4
+    pts = np.random.default_rng(11).uniform(-7, 7, (1000 + 300 * (split % 2), 3))
+    J = ctx.jtensor(pts, precision="fp64")
+    print("case mixed kernel", variant, split, flush=True)
+    ctx.set_mixed_kernel(variant, split)
+    try:
+        Jm = ctx.jtensor(pts, precision="mixed")
+        assert ctx.tuning()[0] == variant
+    finally:
+        ctx.set_mixed_kernel(0)
+    e = rel_err(Jm, J)
+    print("kernel %d/%d per-point error max %.3e p99 %.3e" % (variant, split, e.max(),
+                                                             np.quantile(e, 0.99)))
+    assert e.max() <= MIXED_TOL
+
+
+def test_mixed_open_shell_two_sets(openshell):
+    """alpha and beta sets in one mixed launch (pre-split kernel) against
+    separate FP64 evaluations"""
+    import torch
+    ctx, b, dens = openshell
+    pts = torch.tensor(np.random.default_rng(12).uniform(-8, 8, (777, 3)), dtype=torch.float64,
+                       device="cuda")
+    print("case mixed open shell", flush=True)
+    ctx.set_mixed_kernel(3, 2)
+    try:
+        Jm = ctx.jtensor_multi_device(pts, ["alpha", "beta"], precision="mixed").cpu().numpy()
+    finally:
+        ctx.set_mixed_kernel(0)
+    for k, spin in enumerate(["alpha", "beta"]):
+        J = ctx.jtensor(pts.cpu().numpy(), spin=spin, precision="fp64")
+        assert rel_err(Jm[k], J).max() <= MIXED_TOL
+
+
 def test_beta_on_closed_shell_is_an_error(ring12):
     from gimic_amd.lib import GimicError
     s, ctx = ring12
