@@ -320,8 +320,9 @@ def main():
     J_mixed_sample = out[:: max(1, npts // 65536)].cpu().numpy()
     roof, ktimes = kernel_roofline(ctx, d_pts, out, "mixed", nbf)
     variant, split, tune_ms = ctx.tuning()
+    tuning_log = [[v, sp, round(t, 3)] for v, sp, t in ctx.tuning_log()]
     roof["kernel"] = {1: "contract_bf16x6_kernel", 2: "contract_mx_kernel",
-                      3: "contract_mx2_kernel"}.get(variant, "?")
+                      3: "contract_mx2_kernel<2>", 4: "contract_mx2_kernel<3>"}.get(variant, "?")
     log("timing fp64")
     ms_f64, wall_f64 = time_steps(ctx, d_pts, out, "fp64", args.steps, args.warmup)
     J_f64_sample = out[:: max(1, npts // 65536)].cpu().numpy()
@@ -359,7 +360,8 @@ def main():
                   "tensor_rel_p99": float(np.quantile(e, 0.99)),
                   "tensor_sample_points": int(len(e))},
         "kernel_ms": {k: round(v[0], 3) for k, v in ktimes.items()},
-        "mixed_kernel_tuning": {"chosen": variant, "split": split, "first_chunk_ms": tune_ms},
+        "mixed_kernel_tuning": {"chosen": variant, "split": split,
+                                "candidates_ms": tuning_log},
         "roofline": roof,
         "cpu_baseline": cpu,
         "wall_s": {"mixed": round(wall_mixed, 3), "fp64": round(wall_f64, 3)},

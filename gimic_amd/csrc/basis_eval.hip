@@ -12,11 +12,22 @@
 // (basis.f90:118-136).  Spherical contractions are T_l times the cartesian
 // ones.  Outputs: phi (the contraction's left operand) and its gradient, in
 // FP64 for the FP64 path or FP32 for the mixed path.
+//
+// The split output of the pre-split mixed kernel (phi as three BF16 parts
+// plus FP32 gradients) is produced by basis_eval_staged_kernel: a workgroup
+// owns a tile of 1-8 whole point rows, its threads evaluate (point,
+// contraction) pairs into a zeroed LDS image of those rows, and the rows
+// then leave with 16-byte coalesced stores.  Written directly, every store
+// instruction of a wavefront scatters 64 short values over a whole row.
 #include "device_common.h"
 
 namespace gimic {
 
 __constant__ double c_sph[466];  // T_l, l = 2..5, (2l+1) x ncart(l) row-major
+
+// LDS image of a tile of rows for basis_eval_staged_kernel:
+//   [tile_points][lda/8][3][8] BF16 parts of phi, then [3][tile_points][lda] FP32 gradients
+extern __shared__ __align__(16) unsigned char basis_tile[];
 
 struct Comp {
     signed char e[21][3];
@@ -81,6 +92,22 @@ __device__ __forceinline__ void store(const EvalArgs &a, size_t row, int f, doub
         a.grad32[i] = (float)gx;
         a.grad32[plane + i] = (float)gy;
         a.grad32[2 * plane + i] = (float)gz;
+    } else if (MODE == 3) {
+        // into the LDS tile: `row` is the point's row within the tile
+        const int tp = a.tile_points, lda = a.lda;
+        const float x = (float)v;
+        const __bf16 hi = (__bf16)x;
+        const float r1 = x - (float)hi;
+        const __bf16 mi = (__bf16)r1;
+        const __bf16 lo = (__bf16)(r1 - (float)mi);
+        __bf16 *d = (__bf16 *)basis_tile + row * 3 * (size_t)lda + (f >> 3) * 24 + (f & 7);
+        d[0] = hi;
+        d[8] = mi;
+        d[16] = lo;
+        float *gr = (float *)(basis_tile + (size_t)tp * lda * 6) + row * (size_t)lda + f;
+        gr[0] = (float)gx;
+        gr[(size_t)tp * lda] = (float)gy;
+        gr[2 * (size_t)tp * lda] = (float)gz;
     } else {
         // value as three BF16 parts, x = h + m + l exactly
         const float x = (float)v;
@@ -149,16 +176,18 @@ __device__ void eval_shell(const EvalArgs &a, size_t row, int fout, const double
     }
 }
 
-template <int MODE, bool TM, bool SPH>
+// LMAX: the largest l the kernel instance handles (register use grows with
+// it; the launcher picks the instance for the basis)
+template <int MODE, bool TM, bool SPH, int LMAX = 5>
 __device__ void dispatch_l(const EvalArgs &a, size_t row, const DevShell &sh, const double rr[3],
                            double g, double h) {
     switch (sh.l) {
         case 0: eval_shell<MODE, 0, TM, SPH>(a, row, sh.fout, rr, g, h); break;
         case 1: eval_shell<MODE, 1, TM, SPH>(a, row, sh.fout, rr, g, h); break;
-        case 2: eval_shell<MODE, 2, TM, SPH>(a, row, sh.fout, rr, g, h); break;
-        case 3: eval_shell<MODE, 3, TM, SPH>(a, row, sh.fout, rr, g, h); break;
-        case 4: eval_shell<MODE, 4, TM, SPH>(a, row, sh.fout, rr, g, h); break;
-        default: eval_shell<MODE, 5, TM, SPH>(a, row, sh.fout, rr, g, h); break;
+        case 2: if (LMAX >= 2) eval_shell<MODE, (LMAX >= 2 ? 2 : 0), TM, SPH>(a, row, sh.fout, rr, g, h); break;
+        case 3: if (LMAX >= 3) eval_shell<MODE, (LMAX >= 3 ? 3 : 0), TM, SPH>(a, row, sh.fout, rr, g, h); break;
+        case 4: if (LMAX >= 4) eval_shell<MODE, (LMAX >= 4 ? 4 : 0), TM, SPH>(a, row, sh.fout, rr, g, h); break;
+        default: if (LMAX >= 5) eval_shell<MODE, (LMAX >= 5 ? 5 : 0), TM, SPH>(a, row, sh.fout, rr, g, h); break;
     }
 }
 
@@ -201,6 +230,60 @@ __global__ __launch_bounds__(256) void basis_eval_kernel(EvalArgs a) {
     }
 }
 
+// Split output through LDS (see the file comment).  Grid: rows / tile_points
+// workgroups; tile_points divides 128 and the LDS image is
+// tile_points * lda * 18 bytes.
+template <int LMAX>
+__global__ __launch_bounds__(256) void basis_eval_staged_kernel(EvalArgs a) {
+    const int tid = threadIdx.x, tp = a.tile_points, lda = a.lda;
+    const size_t row0 = (size_t)blockIdx.x * tp;
+    const size_t phi_bytes = (size_t)tp * lda * 6, grad_bytes = (size_t)tp * lda * 12;
+    {
+        uint4 *z = (uint4 *)basis_tile;
+        const size_t n = (phi_bytes + grad_bytes) / 16;
+        for (size_t i = tid; i < n; i += 256) z[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+    const int items = tp * a.nshell;
+    for (int it = tid; it < items; it += 256) {
+        const int lp = it / a.nshell;
+        const size_t row = row0 + lp;
+        if (row >= (size_t)a.npts) continue;  // padding rows stay zero
+        const DevShell sh = a.shells[it % a.nshell];
+        const double *r = a.pts + 3 * row;
+        const double *R = a.coords + 3 * sh.atom;
+        double rr[3] = {r[0] - R[0], r[1] - R[1], r[2] - R[2]};
+        const double s = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
+        if (sqrt(s) > sh.radius) continue;  // screened: zeros already in place
+        double g = 0.0, h = 0.0;
+        for (int p = 0; p < sh.nprim; p++) {
+            const double xp = a.exps[sh.poff + p];
+            const double q = a.ncc[sh.poff + p] * exp(-xp * s);
+            g += q;
+            h += xp * q;
+        }
+        if (a.turbomole) {
+            if (a.spherical) dispatch_l<3, true, true, LMAX>(a, lp, sh, rr, g, h);
+            else dispatch_l<3, true, false, LMAX>(a, lp, sh, rr, g, h);
+        } else {
+            if (a.spherical) dispatch_l<3, false, true, LMAX>(a, lp, sh, rr, g, h);
+            else dispatch_l<3, false, false, LMAX>(a, lp, sh, rr, g, h);
+        }
+    }
+    __syncthreads();
+    // rows row0 .. row0 + tp - 1 are contiguous in both outputs
+    {
+        const uint4 *src = (const uint4 *)basis_tile;
+        uint4 *dst = (uint4 *)(a.phis + row0 * 3 * (size_t)lda);
+        for (size_t i = tid; i < phi_bytes / 16; i += 256) dst[i] = src[i];
+    }
+    for (int d = 0; d < 3; d++) {
+        const uint4 *src = (const uint4 *)(basis_tile + phi_bytes + (size_t)d * tp * lda * 4);
+        uint4 *dst = (uint4 *)(a.grad32 + (size_t)d * a.gplane + row0 * lda);
+        for (size_t i = tid; i < (size_t)tp * lda / 4; i += 256) dst[i] = src[i];
+    }
+}
+
 hipError_t upload_basis_tables(const double *sph, int nsph, hipStream_t s) {
     if (nsph > 466) return hipErrorInvalidValue;
     return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sph), sph, sizeof(double) * nsph, 0,
@@ -215,8 +298,32 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
         hipLaunchKernelGGL(basis_eval_kernel<0>, dim3(nblk), dim3(256), 0, s, a);
     else if (mode == 1)
         hipLaunchKernelGGL(basis_eval_kernel<1>, dim3(nblk), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL(basis_eval_kernel<2>, dim3(nblk), dim3(256), 0, s, a);
+    else {
+        // largest tile of 1, 2, 4 or 8 rows whose LDS image fits in 72 KiB
+        // (two workgroups per CU)
+        EvalArgs b = a;
+        const size_t row_bytes = (size_t)a.lda * 18;
+        int tp = 8;
+        while (tp > 1 && tp * row_bytes > 72 * 1024) tp >>= 1;
+        if (row_bytes > 160 * 1024) return hipErrorInvalidValue;
+        b.tile_points = tp;
+        const size_t lds = tp * row_bytes;
+        if (a.rows % tp || a.lmax < 0 || a.lmax > kDevMaxL) return hipErrorInvalidValue;
+        auto launch = [&](auto kernel) -> hipError_t {
+            hipError_t e = hipFuncSetAttribute((const void *)kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)std::max<size_t>(lds, 72 * 1024));
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(kernel, dim3((unsigned)(a.rows / tp)), dim3(256), lds, s, b);
+            return hipGetLastError();
+        };
+        hipError_t e;
+        if (a.lmax <= 1) e = launch(basis_eval_staged_kernel<1>);
+        else if (a.lmax == 2) e = launch(basis_eval_staged_kernel<2>);
+        else if (a.lmax == 3) e = launch(basis_eval_staged_kernel<3>);
+        else e = launch(basis_eval_staged_kernel<5>);
+        if (e != hipSuccess) return e;
+    }
     return hipGetLastError();
 }
 

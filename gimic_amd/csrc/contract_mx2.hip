@@ -10,11 +10,13 @@
 //   * the right operand (constant per density) is split once on the host into
 //     44 KiB blocks per (density set, N-tile, K-step) that are byte images of
 //     the kernel's LDS buffer, [plane][k-group][column][8].
-// The K-loop therefore has no conversion work: each K-step streams the next
-// block into the other LDS buffer with gfx950's direct global-to-LDS loads
+// The K-loop therefore has no conversion work: each K-step streams a later
+// block into a free LDS buffer with gfx950's direct global-to-LDS loads
 // (global_load_lds_dwordx4: no VGPRs, no ds_write), loads the next A
 // fragments, and runs 168 MFMAs per wavefront against B fragments that are
-// read from LDS one column block ahead.
+// read from LDS one column block ahead.  NBUF = 2 streams block ks+1 during
+// K-step ks; NBUF = 3 streams block ks+2 (154 KiB of LDS), so a block has
+// two K-steps to arrive from the Infinity Cache.
 //
 // Tile: 128 point rows per workgroup (4 wavefronts x 32 rows), 224 columns
 // per N-tile (7 matrices x 32 functions), 2 x 14 MFMA blocks per wavefront.
@@ -41,7 +43,7 @@ constexpr int kBlk = kMx2BlockElems;        // BF16 per block / LDS buffer
 constexpr int kPlane = 4 * kCols * 8;       // BF16 per part plane of a block
 constexpr int kPieces = kBlk * 2 / 1024;    // 1 KiB pieces per block (44)
 constexpr int kPiecesPerWave = kPieces / 4; // 11
-constexpr size_t kLdsB = 2 * (size_t)kBlk * 2;
+constexpr size_t kBufBytes = (size_t)kBlk * 2;
 constexpr size_t kLdsQ = (size_t)kRows * kNQ * 8;
 static_assert(kPieces % 4 == 0, "block must split evenly over 4 wavefronts");
 static_assert(3 * kPlane <= kBlk, "block too small");
@@ -73,10 +75,18 @@ __device__ __forceinline__ v4f mfma6(const v8bf (&a)[3], const v8bf (&b)[3], v4f
 
 }  // namespace
 
+// s_waitcnt immediates (gfx9 encoding: vmcnt [3:0] and [15:14], expcnt
+// [6:4], lgkmcnt [11:8]): everything, or all but the last kPiecesPerWave
+// vector-memory operations (the block copy issued last in a K-step)
+constexpr int kWaitAll = 0;
+constexpr int kWaitAllButBlock = (kPiecesPerWave & 15) | ((kPiecesPerWave >> 4) << 14) | (7 << 4);
+static_assert(kPiecesPerWave < 64, "vmcnt field");
+
+template <int NBUF>
 __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char *bbuf = smem;
-    double *qs = (double *)(smem + kLdsB);
+    double *qs = (double *)(smem + NBUF * kBufBytes);
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -111,12 +121,13 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
             for (int cb = 0; cb < kCB; cb++) acc[rb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
 
         stage_block(Bnt, bbuf, wave, lane);
+        if (NBUF == 3 && nks > 1) stage_block(Bnt + kBlk, bbuf + kBufBytes, wave, lane);
         v8bf an[kRB][3];
 #pragma unroll
         for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
             for (int p = 0; p < 3; p++) an[rb][p] = *(const v8bf *)(Aw + rb * 16 * arow + p * 8);
-        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_s_waitcnt(kWaitAll);
         __syncthreads();
 
         for (int ks = 0; ks < nks; ks++) {
@@ -125,16 +136,27 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
             for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
                 for (int p = 0; p < 3; p++) as[rb][p] = an[rb][p];
+            // A fragments of the next K-step first, then the block copy, so
+            // that waiting for "all but the last copy" covers the A loads
             if (ks + 1 < nks) {
-                stage_block(Bnt + (size_t)(ks + 1) * kBlk, bbuf + (size_t)((ks + 1) & 1) * kBlk * 2,
-                            wave, lane);
 #pragma unroll
                 for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
                     for (int p = 0; p < 3; p++)
                         an[rb][p] = *(const v8bf *)(Aw + rb * 16 * arow + (size_t)(ks + 1) * 96 + p * 8);
             }
-            const __bf16 *cur = (const __bf16 *)(bbuf + (size_t)(ks & 1) * kBlk * 2);
+            __builtin_amdgcn_sched_barrier(0);
+            // block streamed during this K-step.  With NBUF = 3 the copy is
+            // issued every K-step (the last two repeat block nks-1 into a
+            // buffer nobody reads again) so that the count of outstanding
+            // loads is the same on every iteration.
+            const int ahead = ks + NBUF - 1;
+            const bool copy = NBUF == 3 || ahead < nks;
+            if (copy)
+                stage_block(Bnt + (size_t)min(ahead, nks - 1) * kBlk,
+                            bbuf + (size_t)(ahead % NBUF) * kBufBytes, wave, lane);
+            __builtin_amdgcn_sched_barrier(0);
+            const __bf16 *cur = (const __bf16 *)(bbuf + (size_t)(ks % NBUF) * kBufBytes);
             v8bf bf[2][3];
             {
                 const int off = (g * kCols + c) * 8;
@@ -155,11 +177,21 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
                 for (int rb = 0; rb < kRB; rb++) acc[rb][cb] = mfma6(as[rb], bf[sl], acc[rb][cb]);
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // this wavefront's block copy and A loads have landed; after the
-            // barrier every wavefront's copy is visible and nobody reads the
-            // buffer that the next K-step overwrites
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
+            // this wavefront's A loads and the copy of block ks+1 have landed
+            // (with NBUF = 3 the copy of block ks+2 may still be in flight);
+            // after the barrier every wavefront's copy of block ks+1 is
+            // visible and nobody reads the buffer the next copy overwrites
+            // (a bare s_barrier: __syncthreads would add a fence that waits
+            // for every outstanding load, the early copy included)
+            if (NBUF == 3) __builtin_amdgcn_s_waitcnt(kWaitAllButBlock);
+            else __builtin_amdgcn_s_waitcnt(kWaitAll);
+            __builtin_amdgcn_s_barrier();
+        }
+        // no copy may still be writing a buffer that the next N-tile's
+        // first copies reuse
+        if (NBUF == 3) {
+            __builtin_amdgcn_s_waitcnt(kWaitAll);
+            __builtin_amdgcn_s_barrier();
         }
 
         // epilogue: lane (g, c) holds Y[row 4g + r][col c] of every block;
@@ -221,21 +253,26 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
     }
 }
 
-hipError_t launch_contract_mx2(const ContractArgs &a, hipStream_t s) {
-    static bool attr = false;
-    const size_t lds = kLdsB + kLdsQ;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)contract_mx2_kernel,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, hipStream_t s) {
+    static bool attr[2] = {false, false};
+    if (nbuf != 2 && nbuf != 3) return hipErrorInvalidValue;
+    const void *fn = nbuf == 2 ? (const void *)contract_mx2_kernel<2>
+                               : (const void *)contract_mx2_kernel<3>;
+    const size_t lds = nbuf * kBufBytes + kLdsQ;
+    if (!attr[nbuf - 2]) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
-        attr = true;
+        attr[nbuf - 2] = true;
     }
     if (a.rows % kRows || a.lda % 32 || a.ncol_pad != a.lda || a.ntiles * kNu != a.ncol_pad)
         return hipErrorInvalidValue;
     const long long groups = (a.rows / kRows + 7) / 8;
     const long long nwg = 8 * groups * a.nsplit * a.nmatset;
     if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(contract_mx2_kernel, dim3((unsigned)nwg), dim3(256), lds, s, a);
+    if (nbuf == 2)
+        hipLaunchKernelGGL(contract_mx2_kernel<2>, dim3((unsigned)nwg), dim3(256), lds, s, a);
+    else
+        hipLaunchKernelGGL(contract_mx2_kernel<3>, dim3((unsigned)nwg), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 

@@ -68,6 +68,8 @@ struct EvalArgs {
     // split basis values for the pre-split contraction: each FP32 value x as
     // three BF16 parts x = h + m + l (exact), layout [row][lda/8][3][8]
     __bf16 *phis;
+    int tile_points;     // points per workgroup of the LDS-staged kernel (set by the launcher)
+    int lmax;            // largest shell l of the basis
 };
 hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s);
 hipError_t upload_basis_tables(const double *sph, int nsph, hipStream_t s);
@@ -94,7 +96,7 @@ int contract_mx_rows();  // row tile of launch_contract_mx (a multiple of kWgRow
 int contract_mx_cols();  // functions per N-tile of launch_contract_mx
 // Pre-split variant: both operands arrive as BF16 parts, the right operand in
 // blocks that are byte images of the kernel's LDS buffer (see contract_mx2.hip)
-hipError_t launch_contract_mx2(const ContractArgs &a, hipStream_t s);
+hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, hipStream_t s);  // nbuf 2 or 3
 constexpr int kMx2Cols = 224;                         // 7 matrices x 32 functions
 constexpr int kMx2BlockElems = 22528;                 // 3*4*224*8 = 21504, padded to 44 KiB
 

@@ -8,8 +8,12 @@
 // The learned log amplitude y_9 lets the network follow a field whose size
 // changes by orders of magnitude while y_0..8 stay of order one.
 // Training: BF16 operands on MFMA (v_mfma_f32_16x16x32_bf16) with FP32
-// accumulation, FP32 master weights and Adam; loss = mean over points of
-// sum_i (y_i e^{y_9} - t_i)^2 / (|t|^2 + eps^2) with t = J / out_scale.
+// accumulation, FP32 master weights and Adam.  With t = J / out_scale,
+// n = sqrt(|t|^2 + eps^2) and u = t / n, the loss is the mean over points of
+//   sum_i (y_i - u_i)^2 + (y_9 - log n)^2
+// (direction and log magnitude fitted separately, so every point weighs the
+// same whatever the size of its current; the prediction y e^{y_9} then has
+// relative error ~ |y - u| + |y_9 - log n|).
 // Inference: one fused kernel per 64-point tile, activations in LDS.
 //
 // The generic GEMM below takes arbitrary element strides for both operands
@@ -131,9 +135,8 @@ __global__ void mlp_bias_silu_kernel(float *Z, const float *b, __bf16 *H, long l
     H[i] = (__bf16)silu(z);
 }
 
-// output layer: Y = Z + b; loss terms and dY (fp32 and bf16): with
-// q_i = y_i e^{y_9} and d_i = q_i - t_i, dL/dy_i = s d_i e^{y_9} and
-// dL/dy_9 = s sum_i d_i q_i, s = 2 w / npts
+// output layer: Y = Z + b; loss terms and dY (fp32 and bf16):
+// dL/dy_i = 2 (y_i - u_i) / npts, dL/dy_9 = 2 (y_9 - log n) / npts
 __global__ void mlp_loss_kernel(const float *Z, const float *b, const double *target,
                                 const float *out_scale, float eps2, long long npts, int NO,
                                 float *dY, __bf16 *dYb, float *loss_acc) {
@@ -141,30 +144,26 @@ __global__ void mlp_loss_kernel(const float *Z, const float *b, const double *ta
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     float l = 0.f;
     if (p < npts) {
-        float t[9], y[9], tn = 0.f;
+        float t[9], y[9], tn = eps2;
 #pragma unroll
         for (int i = 0; i < 9; i++) {
             t[i] = (float)target[9 * p + i] / out_scale[i];
             y[i] = Z[p * NO + i] + b[i];
             tn += t[i] * t[i];
         }
-        const float z9 = Z[p * NO + 9] + b[9];
-        const float e = amp(z9);
-        const float w = 1.f / (tn + eps2);
-        const float s = 2.f * w / (float)npts;
-        float dz = 0.f;
+        const float inv = rsqrtf(tn);
+        const float dz = (Z[p * NO + 9] + b[9]) - 0.5f * __logf(tn);
+        const float s = 2.f / (float)npts;
+        l = dz * dz;
 #pragma unroll
         for (int i = 0; i < 9; i++) {
-            const float q = y[i] * e;
-            const float d = q - t[i];
-            l += w * d * d;
-            dz += d * q;
-            dY[p * NO + i] = s * d * e;
-            dYb[p * NO + i] = (__bf16)(s * d * e);
+            const float d = y[i] - t[i] * inv;
+            l += d * d;
+            dY[p * NO + i] = s * d;
+            dYb[p * NO + i] = (__bf16)(s * d);
         }
-        dz = z9 < 60.f ? s * dz : 0.f;  // the amplitude is capped at e^60
-        dY[p * NO + 9] = dz;
-        dYb[p * NO + 9] = (__bf16)dz;
+        dY[p * NO + 9] = s * dz;
+        dYb[p * NO + 9] = (__bf16)(s * dz);
         for (int i = 10; i < NO; i++) {
             dY[p * NO + i] = 0.f;
             dYb[p * NO + i] = (__bf16)0.f;

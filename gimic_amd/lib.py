@@ -19,7 +19,7 @@ EXPORTS = [
     "gimic_set_screening", "gimic_calc_jtensor", "gimic_calc_jvector", "gimic_calc_modj",
     "gimic_get_gauss_points", "mkgausspoints", "gimic_last_error", "gimic_ctx_create_files",
     "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
-    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
+    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_tuning_log", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
     "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_sph_transform", "gimic_quadrature",
     "gimic_selftest_mfma_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_set_stream", "gimic_mlp_info",
     "gimic_mlp_params", "gimic_mlp_loss_grad", "gimic_mlp_train_step", "gimic_mlp_predict",
@@ -75,6 +75,7 @@ def load():
         "gimic_ctx_set_timing": [P, I],
         "gimic_ctx_get_timing": [P, P, P],
         "gimic_ctx_tuning": [P, P, P, P],
+        "gimic_ctx_tuning_log": [P, P, P, P],
         "gimic_ctx_set_mixed_kernel": [P, I, I],
         "gimic_ctx_set_graph": [P, I],
         "gimic_ctx_graph_stats": [P, P, P],
@@ -214,16 +215,24 @@ class Context:
 
     def tuning(self):
         """(chosen mixed kernel variant, its N-split count, [best ms of
-        variants 1, 2, 3])"""
+        variants 1 to 4])"""
         v, sp = ctypes.c_int(), ctypes.c_int()
-        ms = np.zeros(3)
+        ms = np.zeros(4)
         check(self._lib.gimic_ctx_tuning(self._h, ctypes.byref(v), ctypes.byref(sp), ptr(ms)))
         return v.value, sp.value, ms.tolist()
 
+    def tuning_log(self):
+        """[(variant, split, ms)] of every candidate the tuner timed"""
+        n = ctypes.c_int(32)
+        codes = np.zeros(32, dtype=np.int32)
+        ms = np.zeros(32)
+        check(self._lib.gimic_ctx_tuning_log(self._h, ctypes.byref(n), ptr(codes), ptr(ms)))
+        return [(int(c) // 100, int(c) % 100, float(t)) for c, t in zip(codes[:n.value], ms)]
+
     def set_mixed_kernel(self, variant=0, split=0):
         """fix the mixed contraction kernel (1 register-tiled, 2 LDS-tiled,
-        3 pre-split with `split` N-ranges, 0 = by workgroup count) or 0 for
-        automatic selection"""
+        3/4 pre-split double/triple buffered with `split` N-ranges, 0 = by
+        workgroup count) or 0 for automatic selection"""
         check(self._lib.gimic_ctx_set_mixed_kernel(self._h, int(variant), int(split)))
 
     def set_graph(self, enable=True):

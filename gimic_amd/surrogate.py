@@ -189,11 +189,11 @@ class Surrogate:
 
     # -- training -----------------------------------------------------------
     def fit(self, n_train=2_000_000, steps=4000, batch=16384, lr=2e-3, lr_min=2e-5, eps_frac=1e-3,
-            holdout=0.01, log_every=0):
+            holdout=0.01, near_frac=0.5, near_sigma=1.0, log_every=0):
         """sample, label with the exact kernel, train, validate per cell"""
         import torch
         t0 = time.perf_counter()
-        pts = self.sample(n_train, seed=self.seed)
+        pts = self.sample(n_train, near_frac=near_frac, near_sigma=near_sigma, seed=self.seed)
         J = self.ctx.jtensor_device(pts, spin=self.spin, precision=self.precision)
         torch.cuda.synchronize()
         t_label = time.perf_counter() - t0

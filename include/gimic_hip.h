@@ -127,12 +127,16 @@ int gimic_ctx_get_timing(gimic_ctx *ctx, double *ms, long long *launches);
 /* Mixed-precision kernel selection (no counterpart in the reference): the
  * first mixed evaluation of a context times basis evaluation + contraction
  * on its first chunk for the register-tiled (1), the LDS-tiled (2) and the
- * pre-split (3) kernels, the last with N-split counts 2, 4, 8 and 16, and
- * keeps the fastest.  Returns the choice (variant -1 before the first mixed
- * call), its split count (0: chosen by workgroup count) and the best time
- * in ms of each variant (ms[3]). */
+ * pre-split kernels with double (3) or triple (4) buffered staging, the
+ * last two with N-split counts 8, 16 and 64, and keeps the fastest.
+ * Returns the choice (variant -1 before the first mixed call), its split
+ * count (0: chosen by workgroup count) and the best time in ms of each
+ * variant (ms[4]). */
 int gimic_ctx_tuning(const gimic_ctx *ctx, int *variant, int *split, double *ms);
-/* Fix the mixed-precision kernel (1, 2 or 3; for 3 also its N-split count,
+/* Every candidate the tuner timed: codes[k] = variant * 100 + split count,
+ * ms[k] its best time.  *n: capacity in, number of candidates out. */
+int gimic_ctx_tuning_log(const gimic_ctx *ctx, int *n, int *codes, double *ms);
+/* Fix the mixed-precision kernel (1 to 4; for 3 and 4 also the N-split count,
  * 0 = by workgroup count) or return to automatic selection (variant 0: the
  * next mixed evaluation tunes again). */
 int gimic_ctx_set_mixed_kernel(gimic_ctx *ctx, int variant, int split);

@@ -11,7 +11,8 @@ l: W_l[n_out][k_in] row-major, then b_l[n_out]):
     h_l = SiLU(W_l h_{l-1} + b_l), l = 0..3                          width 128
     y  = W_4 h_3 + b_4                     16 rows: 9 used + log amplitude y_9
     J  = y[:9] * exp(min(y_9, 60)) * out_scale
-Loss: mean_p sum_i (y_i e^{y_9} - t_i)^2 / (|t|^2 + eps^2), t = J_target / out_scale.
+Loss, with t = J_target / out_scale, n = sqrt(|t|^2 + eps^2), u = t / n:
+    mean_p [ sum_i (y_i - u_i)^2 + (y_9 - log n)^2 ]
 
 `bf16=True` rounds every GEMM operand to bfloat16 (round to nearest even)
 where the device kernels do: features, weights, hidden activations, dY and
@@ -102,15 +103,13 @@ def loss_grad(params, freqs, center_scale, out_scale, r, target, eps, bf=False):
     y, (xs, zs) = forward(params, freqs, center_scale, r, bf)
     P = len(y)
     t = np.asarray(target, dtype=np.float64).reshape(P, 9) / np.asarray(out_scale, dtype=np.float64)
-    w = 1.0 / ((t * t).sum(axis=1) + eps * eps)
-    e = amp(y)
-    q = y[:, :NOUT] * e[:, None]
-    d = q - t
-    loss = float((w * (d * d).sum(axis=1)).mean())
-    s = 2.0 * w / P
+    n = np.sqrt((t * t).sum(axis=1) + eps * eps)
+    d = y[:, :NOUT] - t / n[:, None]
+    dz = y[:, NOUT] - np.log(n)
+    loss = float(((d * d).sum(axis=1) + dz * dz).mean())
     dy = np.zeros((P, NO))
-    dy[:, :NOUT] = s[:, None] * d * e[:, None]
-    dy[:, NOUT] = s * (d * q).sum(axis=1) * (y[:, NOUT] < 60.0)
+    dy[:, :NOUT] = 2.0 * d / P
+    dy[:, NOUT] = 2.0 * dz / P
     lay, n = layout()
     grad = np.zeros(n)
     g = dy
