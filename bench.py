@@ -237,46 +237,57 @@ def bench_surrogate(args):
     system = synth.cage60()
     ctx = system.context(screening=1e-8)
     half = 7.68
+    # the network covers points farther than `exclusion` bohr from every
+    # nucleus (the field there is smooth enough to learn); nearer points and
+    # cells that fail the held-out tolerance go to the exact kernel
     sg = Surrogate(ctx, np.asarray(system.coords), -half * np.ones(3), half * np.ones(3),
-                   tol=1e-3, ncell=8, sigma=args.sigma, seed=3)
-    st = sg.fit(n_train=2_000_000, steps=args.train_steps, batch=16384, lr=2e-3)
+                   tol=1e-3, ncell=8, sigma=args.sigma, seed=3, exclusion=args.exclusion)
+    st = sg.fit(n_train=2_000_000, steps=args.train_steps, batch=16384, lr=2e-3, near_frac=0.0)
     st.pop("loss_history", None)
     ax = torch.linspace(-half, half, 256, dtype=torch.float64, device="cuda")
     z, y, x = torch.meshgrid(ax, ax, ax, indexing="ij")
     grid = torch.stack([x.ravel(), y.ravel(), z.ravel()], 1)
-    out = torch.empty((grid.shape[0], 9), dtype=torch.float64, device="cuda")
-    sg.evaluate(grid, out=out)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    J, served = sg.evaluate(grid, out=out)
-    torch.cuda.synchronize()
-    t_h = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    sg.net.predict(grid, out=out)
-    torch.cuda.synchronize()
-    t_net = time.perf_counter() - t0
-    J, served = sg.evaluate(grid, out=out)
-    judge = sg.judge(grid, J, served)
+    n = grid.shape[0]
+    out = torch.empty((n, 9), dtype=torch.float64, device="cuda")
     sub = grid[::8].contiguous()
+    ctx.jtensor_device(sub, precision="mixed")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ctx.jtensor_device(sub, precision="mixed")
     torch.cuda.synchronize()
-    t_ex = (time.perf_counter() - t0) * 8
+    t_ex = (time.perf_counter() - t0) * 8  # exact-only time for the full grid
+    t0 = time.perf_counter()
+    sg.net.predict(grid, out=out)
+    torch.cuda.synchronize()
+    t_net = time.perf_counter() - t0
+    per_tol = {}
+    for tol in (1e-3, 1e-2, 3e-2):
+        sg.set_tolerance(tol)
+        sg.evaluate(grid, out=out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J, served = sg.evaluate(grid, out=out)
+        torch.cuda.synchronize()
+        t_h = time.perf_counter() - t0
+        per_tol[f"{tol:g}"] = {"served_fraction": round(float(served.float().mean()), 4),
+                               "points_per_s": round(n / t_h, 1),
+                               "gain_over_exact": round(t_ex / t_h, 3),
+                               "judge": sg.judge(grid, J, served)}
     ctx.close()
-    n = grid.shape[0]
+    head = per_tol["0.001"]
     return {
         "metric": "visualisation-grid J-tensor points/s with learned surrogate + exact fallback",
-        "value": round(n / t_h, 1), "unit": "points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": round(1e3 * t_h, 3), "higher_is_better": True,
+        "value": head["points_per_s"], "unit": "points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "higher_is_better": True,
         "dtype": "bf16 MFMA (fp32 master weights); fallback fp32 (bf16x6 MFMA)",
         "data": "synthetic",
         "config": {"workload": "cage60 (nbf 1800) on a 256^3 grid over [-7.68, 7.68]^3 bohr; "
-                               "2M exact-kernel training points, 1% held out, 8^3 cells",
+                               "2M exact-kernel training points beyond %.1f bohr of every "
+                               "nucleus, 1%% held out, 8^3 cells" % args.exclusion,
                    "points_per_step": n},
-        "served_fraction": float(served.float().mean()),
+        "tolerance_1e-3": head, "by_tolerance": per_tol,
         "exact_only_points_per_s": round(n / t_ex, 1), "network_only_points_per_s": round(n / t_net, 1),
-        "gain_over_exact": round(t_ex / t_h, 3), "training": st, "judge": judge,
+        "training": st,
     }
 
 
@@ -289,8 +300,10 @@ def main():
     ap.add_argument("--points", type=int, default=0, help="limit points (profiling)")
     ap.add_argument("--workload", default="cage60",
                     choices=["cage60", "scan", "openshell", "surrogate"])
-    ap.add_argument("--train-steps", type=int, default=6000, help="surrogate training steps")
-    ap.add_argument("--sigma", type=float, default=3.0, help="surrogate Fourier frequency scale")
+    ap.add_argument("--train-steps", type=int, default=12000, help="surrogate training steps")
+    ap.add_argument("--sigma", type=float, default=1.0, help="surrogate Fourier frequency scale")
+    ap.add_argument("--exclusion", type=float, default=3.0,
+                    help="surrogate: exact kernel within this distance (bohr) of a nucleus")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the workload is sharded by point tiles "
@@ -322,7 +335,9 @@ def main():
     variant, split, tune_ms = ctx.tuning()
     tuning_log = [[v, sp, round(t, 3)] for v, sp, t in ctx.tuning_log()]
     roof["kernel"] = {1: "contract_bf16x6_kernel", 2: "contract_mx_kernel",
-                      3: "contract_mx2_kernel<2>", 4: "contract_mx2_kernel<3>"}.get(variant, "?")
+                      3: "contract_mx2_kernel<2,2>", 4: "contract_mx2_kernel<3,2>",
+                      5: "contract_mx2_kernel<2,4>", 6: "contract_mx3_kernel<2>",
+                      7: "contract_mx3_kernel<3>"}.get(variant, "?")
     log("timing fp64")
     ms_f64, wall_f64 = time_steps(ctx, d_pts, out, "fp64", args.steps, args.warmup)
     J_f64_sample = out[:: max(1, npts // 65536)].cpu().numpy()

@@ -116,11 +116,13 @@ struct gimic_ctx {
     bool timing = false;
     // mixed-precision contraction kernel chosen by timing (-1: not yet tuned;
     // 1 register-tiled, 2 LDS-tiled, 3 and 4 pre-split operands with
-    // direct-to-LDS staging, double or triple buffered) and the timings
-    // (basis evaluation + contraction) that chose it
+    // direct-to-LDS staging, double or triple buffered, 5 pre-split with 256
+    // rows per workgroup, 6 and 7 pre-split on 32x32 MFMA blocks, double or
+    // triple buffered) and the timings (basis evaluation + contraction) that
+    // chose it
     int mixed_variant = -1;
     int mixed_split = 0;  // N-split count of the pre-split kernel (0: by workgroup count)
-    float tune_ms[4] = {0.f, 0.f, 0.f, 0.f};
+    float tune_ms[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     std::vector<std::pair<int, float>> tune_log;  // (variant * 100 + split, ms)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
     double ms[3] = {0.0, 0.0, 0.0};
@@ -375,8 +377,9 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
     // within ~8 GiB (FP64: 32 bytes per point and function)
     // (mixed: FP32 values 4 + split values 6 + FP32 gradients 12 bytes)
     const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 22);
-    // row granularity: the workgroup height of the contraction kernels (128)
-    const size_t gran = prec == GIMIC_PREC_FP64 ? (size_t)kWgRows : (size_t)contract_mx_rows();
+    // row granularity: a multiple of every contraction kernel's workgroup
+    // height (128; 256 for the 256-row pre-split kernel)
+    const size_t gran = prec == GIMIC_PREC_FP64 ? (size_t)kWgRows : 256;
     size_t cap = (size_t)8 << 30;
     size_t chunk = std::max<size_t>(gran, (cap / per_row) / gran * gran);
     chunk = std::min(chunk, round_up((size_t)npts, gran));
@@ -451,7 +454,8 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         };
         auto plan_variant = [&](int variant, int split = -1) {
             if (split < 0) split = c->mixed_split;
-            if (variant >= 3) plan(contract_mx_rows(), contract_mx_cols(), split);
+            if (variant == 5) plan(256, contract_mx_cols(), split);
+            else if (variant >= 3) plan(contract_mx_rows(), contract_mx_cols(), split);
             else if (variant == 2) plan(contract_mx_rows(), contract_mx_cols());
             else plan(kWgRows, kNT);
         };
@@ -460,7 +464,10 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             if (variant == 0) hip_check(launch_contract_f64(ca, st), "contract_f64");
             else if (variant == 1) hip_check(launch_contract_bf16x6(ca, st), "contract_bf16x6");
             else if (variant == 2) hip_check(launch_contract_mx(ca, st), "contract_mx");
-            else hip_check(launch_contract_mx2(ca, variant == 3 ? 2 : 3, st), "contract_mx2");
+            else if (variant == 3) hip_check(launch_contract_mx2(ca, 2, 2, st), "contract_mx2");
+            else if (variant == 4) hip_check(launch_contract_mx2(ca, 3, 2, st), "contract_mx2");
+            else if (variant == 5) hip_check(launch_contract_mx2(ca, 2, 4, st), "contract_mx2");
+            else hip_check(launch_contract_mx3(ca, variant == 6 ? 2 : 3, st), "contract_mx3");
         };
         // basis evaluation output each contraction kernel reads
         auto eval_mode = [](int variant) { return variant == 0 ? 0 : variant >= 3 ? 2 : 1; };
@@ -481,8 +488,8 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             // timed twice and its faster run counts; keep the fastest
             float best = 0.f;
             int best_split = 0;
-            const int cand[][2] = {{1, 0}, {2, 0}, {3, 8}, {3, 16}, {3, 64},
-                                   {4, 8}, {4, 16}, {4, 64}};
+            const int cand[][2] = {{3, 16}, {3, 64}, {4, 16}, {4, 64}, {5, 16}, {5, 64},
+                                   {6, 8}, {6, 16}, {6, 64}, {7, 8}, {7, 16}, {7, 64}};
             c->tune_log.clear();
             for (auto &cv : cand) {
                 const int v = cv[0], sp = cv[1];
@@ -683,7 +690,7 @@ int gimic_ctx_tuning(const gimic_ctx *c, int *variant, int *split, double *ms) {
         if (!c || !variant || !split || !ms) throw Error(GIMIC_EINVAL, "null argument");
         *variant = c->mixed_variant;
         *split = c->mixed_split;
-        for (int k = 0; k < 4; k++) ms[k] = c->tune_ms[k];
+        for (int k = 0; k < 7; k++) ms[k] = c->tune_ms[k];
     });
 }
 
@@ -861,7 +868,7 @@ int gimic_ctx_tuning_log(const gimic_ctx *c, int *n, int *codes, double *ms) {
 int gimic_ctx_set_mixed_kernel(gimic_ctx *c, int variant, int split) {
     return guarded([&] {
         if (!c) throw Error(GIMIC_EINVAL, "null context");
-        if (variant < 0 || variant > 4) throw Error(GIMIC_EINVAL, "mixed kernel must be 0..4");
+        if (variant < 0 || variant > 7) throw Error(GIMIC_EINVAL, "mixed kernel must be 0..7");
         if (split < 0) throw Error(GIMIC_EINVAL, "split count must be >= 0");
         c->mixed_variant = variant == 0 ? -1 : variant;
         c->mixed_split = variant >= 3 ? split : 0;
@@ -894,6 +901,13 @@ int gimic_sph_transform(int l, double *T) {
         }
         auto v = sph_transform(l);
         std::memcpy(T, v.data(), v.size() * sizeof(double));
+    });
+}
+
+int gimic_selftest_mfma32_layout(float *out) {
+    return guarded([&] {
+        if (!out) throw Error(GIMIC_EINVAL, "null argument");
+        hip_check(run_mfma32_layout_probe(out), "mfma32 layout probe");
     });
 }
 

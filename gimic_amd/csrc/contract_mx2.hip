@@ -18,8 +18,11 @@
 // K-step ks; NBUF = 3 streams block ks+2 (154 KiB of LDS), so a block has
 // two K-steps to arrive from the Infinity Cache.
 //
-// Tile: 128 point rows per workgroup (4 wavefronts x 32 rows), 224 columns
-// per N-tile (7 matrices x 32 functions), 2 x 14 MFMA blocks per wavefront.
+// Tile: 64 * KRB point rows per workgroup (4 wavefronts x 16 KRB rows), 224
+// columns per N-tile (7 matrices x 32 functions), KRB x 14 MFMA blocks per
+// wavefront.  KRB = 4 reads every B fragment from LDS once for 4 row blocks
+// (half the LDS traffic per MFMA of KRB = 2) and doubles the work per K-step
+// that hides the next step's loads; it needs 224 accumulator registers.
 //
 // Workgroup order.  Every workgroup re-reads its 128 rows of A once per
 // N-tile, so A traffic is what limits this kernel.  The grid is 1-D and
@@ -34,33 +37,34 @@ namespace gimic {
 
 namespace {
 
-constexpr int kRows = 128;
 constexpr int kNu = 32;
 constexpr int kCols = kMx2Cols;             // 224
 constexpr int kCB = kCols / 16;             // 14 column blocks
-constexpr int kRB = 2;                      // row blocks per wavefront
 constexpr int kBlk = kMx2BlockElems;        // BF16 per block / LDS buffer
 constexpr int kPlane = 4 * kCols * 8;       // BF16 per part plane of a block
 constexpr int kPieces = kBlk * 2 / 1024;    // 1 KiB pieces per block (44)
 constexpr int kPiecesPerWave = kPieces / 4; // 11
 constexpr size_t kBufBytes = (size_t)kBlk * 2;
-constexpr size_t kLdsQ = (size_t)kRows * kNQ * 8;
+constexpr size_t lds_q(int rows) { return (size_t)rows * kNQ * 8; }
 static_assert(kPieces % 4 == 0, "block must split evenly over 4 wavefronts");
 static_assert(3 * kPlane <= kBlk, "block too small");
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// one 44 KiB block into an LDS buffer: wavefront w copies pieces w, w+4, ...
-// (64 lanes x 16 bytes per instruction, LDS base in M0)
+// piece i of this wavefront's share of a 44 KiB block: wavefront w copies
+// pieces w, w+4, ... (64 lanes x 16 bytes per instruction, LDS base in M0)
+__device__ __forceinline__ void stage_piece(const __bf16 *blk, unsigned char *buf, int wave,
+                                            int lane, int i) {
+    const int piece = wave + 4 * i;
+    const unsigned char *src = (const unsigned char *)blk + piece * 1024 + lane * 16;
+    __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(buf + piece * 1024), 16, 0,
+                                     0);
+}
+
 __device__ __forceinline__ void stage_block(const __bf16 *blk, unsigned char *buf, int wave,
                                             int lane) {
 #pragma unroll
-    for (int i = 0; i < kPiecesPerWave; i++) {
-        const int piece = wave + 4 * i;
-        const unsigned char *src = (const unsigned char *)blk + piece * 1024 + lane * 16;
-        __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(buf + piece * 1024), 16,
-                                         0, 0);
-    }
+    for (int i = 0; i < kPiecesPerWave; i++) stage_piece(blk, buf, wave, lane, i);
 }
 
 __device__ __forceinline__ v4f mfma6(const v8bf (&a)[3], const v8bf (&b)[3], v4f acc) {
@@ -82,8 +86,10 @@ constexpr int kWaitAll = 0;
 constexpr int kWaitAllButBlock = (kPiecesPerWave & 15) | ((kPiecesPerWave >> 4) << 14) | (7 << 4);
 static_assert(kPiecesPerWave < 64, "vmcnt field");
 
-template <int NBUF>
+template <int NBUF, int KRB>
 __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
+    constexpr int kRows = 64 * KRB;  // point rows per workgroup
+    constexpr int kRB = KRB;         // row blocks per wavefront
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char *bbuf = smem;
     double *qs = (double *)(smem + NBUF * kBufBytes);
@@ -107,7 +113,7 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
     for (int i = tid; i < kRows * kNQ; i += 256) qs[i] = 0.0;
 
     // lane (g, c) of row block rb: row wave*32 + rb*16 + c, k = 32 ks + 8 g .. + 7
-    const __bf16 *Aw = a.as3 + (row0 + wave * 32 + c) * arow + g * 24;
+    const __bf16 *Aw = a.as3 + (row0 + wave * 16 * kRB + c) * arow + g * 24;
     const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kBlk;
     const int nt0 = split * a.tiles_per_split;
     const int nt1 = min(nt0 + a.tiles_per_split, a.ntiles);
@@ -150,12 +156,14 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
             // issued every K-step (the last two repeat block nks-1 into a
             // buffer nobody reads again) so that the count of outstanding
             // loads is the same on every iteration.
+            // The copy's 11 instructions are spread over the first 11 column
+            // blocks, between MFMA groups: issued back to back they stall
+            // the in-order instruction stream (and the matrix pipe behind
+            // it) while the vector-memory pipe works through them.
             const int ahead = ks + NBUF - 1;
             const bool copy = NBUF == 3 || ahead < nks;
-            if (copy)
-                stage_block(Bnt + (size_t)min(ahead, nks - 1) * kBlk,
-                            bbuf + (size_t)(ahead % NBUF) * kBufBytes, wave, lane);
-            __builtin_amdgcn_sched_barrier(0);
+            const __bf16 *cblk = Bnt + (size_t)min(ahead, nks - 1) * kBlk;
+            unsigned char *cbuf = bbuf + (size_t)(ahead % NBUF) * kBufBytes;
             const __bf16 *cur = (const __bf16 *)(bbuf + (size_t)(ks % NBUF) * kBufBytes);
             v8bf bf[2][3];
             {
@@ -172,6 +180,7 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
                     for (int p = 0; p < 3; p++)
                         bf[sl ^ 1][p] = *(const v8bf *)(cur + p * kPlane + off);
                 }
+                if (copy && cb < kPiecesPerWave) stage_piece(cblk, cbuf, wave, lane, cb);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int rb = 0; rb < kRB; rb++) acc[rb][cb] = mfma6(as[rb], bf[sl], acc[rb][cb]);
@@ -205,7 +214,7 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
         for (int rb = 0; rb < kRB; rb++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int rl = wave * 32 + rb * 16 + 4 * g + r;
+                const int rl = wave * 16 * kRB + rb * 16 + 4 * g + r;
                 const size_t p = row0 + rl;
                 const bool valid = p < (size_t)a.npts;
                 const float rp[3] = {valid ? (float)a.pts[3 * p] : 0.f,
@@ -253,27 +262,31 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
     }
 }
 
-hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, hipStream_t s) {
-    static bool attr[2] = {false, false};
-    if (nbuf != 2 && nbuf != 3) return hipErrorInvalidValue;
-    const void *fn = nbuf == 2 ? (const void *)contract_mx2_kernel<2>
-                               : (const void *)contract_mx2_kernel<3>;
-    const size_t lds = nbuf * kBufBytes + kLdsQ;
-    if (!attr[nbuf - 2]) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+template <int NBUF, int KRB>
+static hipError_t launch_one(const ContractArgs &a, hipStream_t s) {
+    static bool attr = false;
+    constexpr int rows = 64 * KRB;
+    const size_t lds = NBUF * kBufBytes + lds_q(rows);
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)contract_mx2_kernel<NBUF, KRB>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
-        attr[nbuf - 2] = true;
+        attr = true;
     }
-    if (a.rows % kRows || a.lda % 32 || a.ncol_pad != a.lda || a.ntiles * kNu != a.ncol_pad)
+    if (a.rows % rows || a.lda % 32 || a.ncol_pad != a.lda || a.ntiles * kNu != a.ncol_pad)
         return hipErrorInvalidValue;
-    const long long groups = (a.rows / kRows + 7) / 8;
+    const long long groups = (a.rows / rows + 7) / 8;
     const long long nwg = 8 * groups * a.nsplit * a.nmatset;
     if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (nbuf == 2)
-        hipLaunchKernelGGL(contract_mx2_kernel<2>, dim3((unsigned)nwg), dim3(256), lds, s, a);
-    else
-        hipLaunchKernelGGL(contract_mx2_kernel<3>, dim3((unsigned)nwg), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((contract_mx2_kernel<NBUF, KRB>), dim3((unsigned)nwg), dim3(256), lds, s, a);
     return hipGetLastError();
+}
+
+hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, hipStream_t s) {
+    if (nbuf == 2 && krb == 2) return launch_one<2, 2>(a, s);
+    if (nbuf == 3 && krb == 2) return launch_one<3, 2>(a, s);
+    if (nbuf == 2 && krb == 4) return launch_one<2, 4>(a, s);  // 3 buffers + 256-row partials exceed LDS
+    return hipErrorInvalidValue;
 }
 
 }  // namespace gimic

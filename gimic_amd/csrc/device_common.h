@@ -96,7 +96,11 @@ int contract_mx_rows();  // row tile of launch_contract_mx (a multiple of kWgRow
 int contract_mx_cols();  // functions per N-tile of launch_contract_mx
 // Pre-split variant: both operands arrive as BF16 parts, the right operand in
 // blocks that are byte images of the kernel's LDS buffer (see contract_mx2.hip)
-hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, hipStream_t s);  // nbuf 2 or 3
+// nbuf 2 or 3 LDS buffers, krb 2 or 4 row blocks per wavefront (128 or 256
+// rows per workgroup; nbuf 3 only with krb 2)
+hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, hipStream_t s);
+// Pre-split operands on 32x32x16 MFMA blocks, 128 rows per workgroup (contract_mx3.hip)
+hipError_t launch_contract_mx3(const ContractArgs &a, int nbuf, hipStream_t s);
 constexpr int kMx2Cols = 224;                         // 7 matrices x 32 functions
 constexpr int kMx2BlockElems = 22528;                 // 3*4*224*8 = 21504, padded to 44 KiB
 
@@ -122,5 +126,6 @@ hipError_t launch_plane_reduce(const double *contrib, const long long *offs, int
                                double *out, hipStream_t s);
 
 hipError_t run_mfma_layout_probe(double *h_f64, float *h_bf16);
+hipError_t run_mfma32_layout_probe(float *h_out);  // 2048 floats
 
 }  // namespace gimic

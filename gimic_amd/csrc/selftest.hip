@@ -34,6 +34,39 @@ __global__ void mfma_layout_kernel(double *f64, float *bf16) {
     for (int r = 0; r < 4; r++) bf16[256 + l * 4 + r] = f[r];
 }
 
+// 32x32x16 BF16: out[l*16 + e] with A = lane+1, B = 1, then out[1024 + ...]
+// with A = 1, B = lane+1
+__global__ void mfma32_layout_kernel(float *out) {
+    const int l = threadIdx.x;
+    typedef float v16f __attribute__((ext_vector_type(16)));
+    u16x8 lane, one;
+    const uint16_t hl = f32_to_bf16_rn((float)(l + 1)), h1 = f32_to_bf16_rn(1.0f);
+    for (int e = 0; e < 8; e++) {
+        lane[e] = hl;
+        one[e] = h1;
+    }
+    v16f f;
+    for (int e = 0; e < 16; e++) f[e] = 0.f;
+    f = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, lane),
+                                                 __builtin_bit_cast(v8bf, one), f, 0, 0, 0);
+    for (int e = 0; e < 16; e++) out[l * 16 + e] = f[e];
+    for (int e = 0; e < 16; e++) f[e] = 0.f;
+    f = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, one),
+                                                 __builtin_bit_cast(v8bf, lane), f, 0, 0, 0);
+    for (int e = 0; e < 16; e++) out[1024 + l * 16 + e] = f[e];
+}
+
+hipError_t run_mfma32_layout_probe(float *h_out) {
+    float *d = nullptr;
+    hipError_t e = hipMalloc(&d, 2048 * sizeof(float));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mfma32_layout_kernel, dim3(1), dim3(64), 0, 0, d);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(h_out, d, 2048 * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e;
+}
+
 hipError_t run_mfma_layout_probe(double *h_f64, float *h_bf16) {
     double *d64 = nullptr;
     float *d32 = nullptr;

@@ -21,7 +21,7 @@ EXPORTS = [
     "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
     "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_tuning_log", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
     "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_sph_transform", "gimic_quadrature",
-    "gimic_selftest_mfma_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_set_stream", "gimic_mlp_info",
+    "gimic_selftest_mfma_layout", "gimic_selftest_mfma32_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_set_stream", "gimic_mlp_info",
     "gimic_mlp_params", "gimic_mlp_loss_grad", "gimic_mlp_train_step", "gimic_mlp_predict",
     "gimic_mlp_predict_unfused",
 ]
@@ -86,6 +86,7 @@ def load():
         "gimic_sph_transform": [I, P],
         "gimic_quadrature": [D, D, I, I, I, P, P],
         "gimic_selftest_mfma_layout": [P, P],
+        "gimic_selftest_mfma32_layout": [P],
         "gimic_mlp_create": [P, D, P, P, ctypes.c_uint64],
         "gimic_mlp_destroy": [P],
         "gimic_mlp_set_stream": [P, P],
@@ -102,7 +103,8 @@ def load():
         if name.startswith(("gimic_ctx", "gimic_mlp")) or name in ("gimic_jtensor_batch", "gimic_jtensor_batch_device",
                                                     "gimic_jtensor_multi_device", "gimic_flux_planes",
                                                     "gimic_sph_transform", "gimic_quadrature",
-                                                    "gimic_selftest_mfma_layout"):
+                                                    "gimic_selftest_mfma_layout",
+                                                    "gimic_selftest_mfma32_layout"):
             fn.restype = I
         else:
             fn.restype = None
@@ -215,9 +217,9 @@ class Context:
 
     def tuning(self):
         """(chosen mixed kernel variant, its N-split count, [best ms of
-        variants 1 to 4])"""
+        variants 1 to 7, 0 where not timed])"""
         v, sp = ctypes.c_int(), ctypes.c_int()
-        ms = np.zeros(4)
+        ms = np.zeros(7)
         check(self._lib.gimic_ctx_tuning(self._h, ctypes.byref(v), ctypes.byref(sp), ptr(ms)))
         return v.value, sp.value, ms.tolist()
 
@@ -231,8 +233,10 @@ class Context:
 
     def set_mixed_kernel(self, variant=0, split=0):
         """fix the mixed contraction kernel (1 register-tiled, 2 LDS-tiled,
-        3/4 pre-split double/triple buffered with `split` N-ranges, 0 = by
-        workgroup count) or 0 for automatic selection"""
+        3/4 pre-split double/triple buffered, 5 pre-split with 256-row
+        tiles, 6/7 pre-split on 32x32 MFMA blocks double/triple buffered,
+        with `split` N-ranges, 0 = by workgroup count) or 0 for automatic
+        selection"""
         check(self._lib.gimic_ctx_set_mixed_kernel(self._h, int(variant), int(split)))
 
     def set_graph(self, enable=True):

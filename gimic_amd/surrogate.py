@@ -232,6 +232,7 @@ class Surrogate:
         cells = self.cell_index(r_ho)
         nc = self.ncell ** 3
         err, cnt = rel_cell_error(pred, j_ho, cells, nc)
+        self._cell_err, self._cell_cnt = err, cnt
         self.cell_ok = (err <= self.tol) & (cnt >= self.min_holdout)
         tot = float(torch.sqrt(((pred - j_ho) ** 2).sum() / (j_ho ** 2).sum()))
         self.stats = dict(n_train=int(ntr), n_holdout=int(nh), steps=steps, batch=batch,
@@ -242,6 +243,12 @@ class Surrogate:
         return self.stats
 
     # -- evaluation ---------------------------------------------------------
+    def set_tolerance(self, tol):
+        """re-select the served cells for another held-out tolerance"""
+        self.tol = tol
+        self.cell_ok = (self._cell_err <= tol) & (self._cell_cnt >= self.min_holdout)
+        self.stats["cells_served"] = int(self.cell_ok.sum())
+
     def served_mask(self, r):
         return self.cell_ok[self.cell_index(r)] & ~self.near_nucleus(r)
 

@@ -124,19 +124,21 @@ int gimic_ctx_atoms(const gimic_ctx *ctx, double *coords, double *charges);
 int gimic_ctx_set_timing(gimic_ctx *ctx, int on);
 int gimic_ctx_get_timing(gimic_ctx *ctx, double *ms, long long *launches);
 
-/* Mixed-precision kernel selection (no counterpart in the reference): the
- * first mixed evaluation of a context times basis evaluation + contraction
- * on its first chunk for the register-tiled (1), the LDS-tiled (2) and the
- * pre-split kernels with double (3) or triple (4) buffered staging, the
- * last two with N-split counts 8, 16 and 64, and keeps the fastest.
- * Returns the choice (variant -1 before the first mixed call), its split
- * count (0: chosen by workgroup count) and the best time in ms of each
- * variant (ms[4]). */
+/* Mixed-precision kernel selection (no counterpart in the reference).
+ * Kernels: register-tiled (1), LDS-tiled (2), pre-split operands on 16x16
+ * MFMA blocks double (3) or triple (4) buffered with 128 rows per workgroup
+ * or double buffered with 256 rows (5), pre-split on 32x32 MFMA blocks
+ * double (6) or triple (7) buffered.  The first mixed evaluation of a
+ * context times basis evaluation + contraction on its first chunk for
+ * kernels 3 to 7 with N-split counts 8 (6, 7 only), 16 and 64, each twice,
+ * and keeps the fastest.  Returns the choice (variant -1 before the first
+ * mixed call), its split count (0: chosen by workgroup count) and the best
+ * time in ms of each variant (ms[7], 0 for variants not timed). */
 int gimic_ctx_tuning(const gimic_ctx *ctx, int *variant, int *split, double *ms);
 /* Every candidate the tuner timed: codes[k] = variant * 100 + split count,
  * ms[k] its best time.  *n: capacity in, number of candidates out. */
 int gimic_ctx_tuning_log(const gimic_ctx *ctx, int *n, int *codes, double *ms);
-/* Fix the mixed-precision kernel (1 to 4; for 3 and 4 also the N-split count,
+/* Fix the mixed-precision kernel (1 to 7; for 3 to 7 also the N-split count,
  * 0 = by workgroup count) or return to automatic selection (variant 0: the
  * next mixed evaluation tunes again). */
 int gimic_ctx_set_mixed_kernel(gimic_ctx *ctx, int variant, int split);
@@ -238,6 +240,9 @@ int gimic_mlp_predict_unfused(gimic_mlp *net, long long npts, const double *d_r,
  * 512 values each (rows then columns; see gimic_amd/csrc/selftest.hip), so a
  * test can confirm the lane layout the kernels assume. */
 int gimic_selftest_mfma_layout(double *f64, float *bf16);
+/* The same for v_mfma_f32_32x32x16_bf16: 2048 floats, 64 lanes x 16 slots
+ * with A = lane+1 (rows), then with B = lane+1 (columns). */
+int gimic_selftest_mfma32_layout(float *out);
 
 #ifdef __cplusplus
 }

@@ -43,3 +43,25 @@ def test_mfma_layouts_match_kernel_assumptions():
             for r in range(4):
                 assert cols[l, r] == l % 16, (kind, l, r, cols[l, r])
                 assert rows[l, r] == kernel_row(l, r, kind), (kind, l, r, rows[l, r])
+
+
+def row32(l, e):
+    """row of output slot (lane l, element e) of 32x32x16 BF16 as the
+    32x32-block kernel assumes: 8 (e // 4) + 4 (l // 32) + e % 4"""
+    return 8 * (e // 4) + 4 * (l // 32) + e % 4
+
+
+def test_mfma32_layout_matches_kernel_assumptions():
+    """32x32x16 BF16: A lanes feeding row i are i and i+32 (8 k each), B
+    lanes feeding column j are j and j+32"""
+    out = np.zeros(2048, dtype=np.float32)
+    lib.check(lib.load().gimic_selftest_mfma32_layout(lib.ptr(out)))
+    out = out.astype(np.float64)
+    rows = ((out[:1024] / 8.0 - 34.0) / 2.0).reshape(64, 16)
+    cols = ((out[1024:] / 8.0 - 34.0) / 2.0).reshape(64, 16)
+    print("32x32 rows of lanes 0, 32:", rows[[0, 32]].tolist())
+    print("32x32 cols of lanes 0..2:", cols[:3].tolist())
+    for l in range(64):
+        for e in range(16):
+            assert cols[l, e] == l % 32, (l, e, cols[l, e])
+            assert rows[l, e] == row32(l, e), (l, e, rows[l, e])

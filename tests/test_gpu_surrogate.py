@@ -139,3 +139,44 @@ def test_fallback_covers_nuclei_and_unvalidated_cells():
     cells = sg.cell_index(r).cpu().numpy()
     assert np.array_equal(served, ok[cells] & ~sg.near_nucleus(r).cpu().numpy())
     ctx.close()
+
+
+def test_surrogate_gains_over_exact_kernel():
+    """what the trained network gains: on the 60-atom cage a far-field
+    network (exact kernel within 3 bohr of every nucleus) serves the cells
+    whose held-out error is within 5e-2; the hybrid field is faster than the
+    exact kernel alone, and the exact kernel, judging a sample of the served
+    points, confirms the error"""
+    import time
+    import torch
+    from gimic_amd import synth
+    from gimic_amd.surrogate import Surrogate
+    s = synth.cage60()
+    ctx = s.context()
+    half = 7.68
+    sg = Surrogate(ctx, np.asarray(s.coords), -half * np.ones(3), half * np.ones(3), tol=5e-2,
+                   ncell=8, sigma=1.0, seed=3, exclusion=3.0)
+    print("case surrogate gain", flush=True)
+    st = sg.fit(n_train=1_000_000, steps=6000, batch=16384, lr=2e-3, near_frac=0.0)
+    ax = torch.linspace(-half, half, 96, dtype=torch.float64, device="cuda")
+    z, y, x = torch.meshgrid(ax, ax, ax, indexing="ij")
+    grid = torch.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    ctx.jtensor_device(grid, precision="mixed")
+    sg.evaluate(grid)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.jtensor_device(grid, precision="mixed")
+    torch.cuda.synchronize()
+    t_exact = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J, served = sg.evaluate(grid)
+    torch.cuda.synchronize()
+    t_hybrid = time.perf_counter() - t0
+    frac = float(served.float().mean())
+    judge = sg.judge(grid, J, served)
+    print("cells served", st["cells_served"], "grid served %.3f" % frac, "exact %.3f s hybrid %.3f s"
+          % (t_exact, t_hybrid), "judge", judge)
+    assert frac > 0.05
+    assert judge["rel_err"] <= 5e-2
+    assert t_hybrid < t_exact
+    ctx.close()
