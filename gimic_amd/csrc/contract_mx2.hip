@@ -142,24 +142,22 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
             for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
                 for (int p = 0; p < 3; p++) as[rb][p] = an[rb][p];
-            // A fragments of the next K-step first, then the block copy, so
-            // that waiting for "all but the last copy" covers the A loads
-            if (ks + 1 < nks) {
-#pragma unroll
-                for (int rb = 0; rb < kRB; rb++)
-#pragma unroll
-                    for (int p = 0; p < 3; p++)
-                        an[rb][p] = *(const v8bf *)(Aw + rb * 16 * arow + (size_t)(ks + 1) * 96 + p * 8);
-            }
-            __builtin_amdgcn_sched_barrier(0);
-            // block streamed during this K-step.  With NBUF = 3 the copy is
+            // Block streamed during this K-step.  With NBUF = 3 the copy is
             // issued every K-step (the last two repeat block nks-1 into a
             // buffer nobody reads again) so that the count of outstanding
             // loads is the same on every iteration.
-            // The copy's 11 instructions are spread over the first 11 column
-            // blocks, between MFMA groups: issued back to back they stall
-            // the in-order instruction stream (and the matrix pipe behind
-            // it) while the vector-memory pipe works through them.
+            // Vector-memory instructions (the next K-step's A fragments, two
+            // per column block from the first one on, and the block copy's 11
+            // pieces) are spread over the column blocks, between MFMA groups:
+            // issued back to back they stall the in-order instruction stream,
+            // and the matrix pipe behind it, while the vector-memory pipe
+            // works through them.  With NBUF = 3 the copy comes after all A
+            // loads (column blocks 3 to 13), so that waiting for "all but the
+            // last copy" covers the A loads.
+            const bool more = ks + 1 < nks;
+            constexpr int kALoads = 3 * kRB;
+            constexpr int kCopyFrom = NBUF == 3 ? kCB - kPiecesPerWave : 0;
+            static_assert(NBUF != 3 || (kALoads + 1) / 2 <= kCopyFrom, "A loads must precede the copy");
             const int ahead = ks + NBUF - 1;
             const bool copy = NBUF == 3 || ahead < nks;
             const __bf16 *cblk = Bnt + (size_t)min(ahead, nks - 1) * kBlk;
@@ -180,7 +178,13 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
                     for (int p = 0; p < 3; p++)
                         bf[sl ^ 1][p] = *(const v8bf *)(cur + p * kPlane + off);
                 }
-                if (copy && cb < kPiecesPerWave) stage_piece(cblk, cbuf, wave, lane, cb);
+#pragma unroll
+                for (int j = 2 * cb; j < 2 * cb + 2; j++)
+                    if (more && j < kALoads)
+                        an[j / 3][j % 3] = *(const v8bf *)(Aw + (j / 3) * 16 * arow +
+                                                          (size_t)(ks + 1) * 96 + (j % 3) * 8);
+                if (copy && cb >= kCopyFrom && cb - kCopyFrom < kPiecesPerWave)
+                    stage_piece(cblk, cbuf, wave, lane, cb - kCopyFrom);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int rb = 0; rb < kRB; rb++) acc[rb][cb] = mfma6(as[rb], bf[sl], acc[rb][cb]);

@@ -153,16 +153,11 @@ __global__ __launch_bounds__(256, 1) void contract_mx3_kernel(ContractArgs a) {
             for (int h = 0; h < 2; h++)
 #pragma unroll
                 for (int p = 0; p < 3; p++) as[h][p] = an[h][p];
-            if (ks + 1 < nks) {
-#pragma unroll
-                for (int h = 0; h < 2; h++)
-#pragma unroll
-                    for (int p = 0; p < 3; p++)
-                        an[h][p] = *(const v8bf *)(Aw + ((size_t)(ks + 1) * 4 + 2 * h) * 24 + p * 8);
-            }
-            __builtin_amdgcn_sched_barrier(0);
-            // the copy's 11 instructions go two per MFMA group (see
-            // contract_mx2.hip: issued back to back they stall the matrix pipe)
+            // vector-memory instructions spread over the MFMA groups (see
+            // contract_mx2.hip): the 6 A loads three per group in groups 0
+            // and 1, the copy's 11 pieces after them (NBUF = 3: groups 2 to
+            // 5) or two per group (NBUF = 2)
+            const bool more = ks + 1 < nks;
             const int ahead = ks + NBUF - 1;
             const bool copy = NBUF == 3 || ahead < nks;
             const __bf16 *cblk = Bnt + (size_t)min(ahead, nks - 1) * kBlk;
@@ -175,9 +170,21 @@ __global__ __launch_bounds__(256, 1) void contract_mx3_kernel(ContractArgs a) {
             for (int grp = 0; grp < 6; grp++) {
                 const int h = grp / 3, pb = 2 - grp % 3;
                 if (grp + 1 < 6) read_group(bf[(grp + 1) & 1], cur, 2 - (grp + 1) % 3, (grp + 1) / 3, q, c);
+#pragma unroll
+                for (int j = 3 * grp; j < 3 * grp + 3; j++)
+                    if (more && j < 6)
+                        an[j / 3][j % 3] = *(const v8bf *)(Aw + ((size_t)(ks + 1) * 4 + 2 * (j / 3)) * 24 +
+                                                          (j % 3) * 8);
                 if (copy) {
-                    if (2 * grp < kPiecesPerWave) stage_piece(cblk, cbuf, wave, lane, 2 * grp);
-                    if (2 * grp + 1 < kPiecesPerWave) stage_piece(cblk, cbuf, wave, lane, 2 * grp + 1);
+                    if (NBUF == 3) {
+                        // pieces 0-2, 3-5, 6-8, 9-10 in groups 2..5
+#pragma unroll
+                        for (int i = 3 * (grp - 2); i < 3 * (grp - 2) + 3; i++)
+                            if (grp >= 2 && i < kPiecesPerWave) stage_piece(cblk, cbuf, wave, lane, i);
+                    } else {
+                        if (2 * grp < kPiecesPerWave) stage_piece(cblk, cbuf, wave, lane, 2 * grp);
+                        if (2 * grp + 1 < kPiecesPerWave) stage_piece(cblk, cbuf, wave, lane, 2 * grp + 1);
+                    }
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
