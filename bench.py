@@ -91,6 +91,21 @@ def kernel_roofline(ctx, d_pts, out, precision, nbf):
                 flop_per_launch=flop / launches), t
 
 
+def measured_traffic(kernel, split, points_per_launch):
+    """HBM-side bytes per launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE of the same kernel and split on this workload,
+    calibrated against a 1 GiB copy), or None when none matches"""
+    path = os.path.join(ROOT, "profiles", "contract_mx2_traffic_cage60.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and f[0] == "traffic_bytes_per_point" and f[1] == kernel and \
+                f[2] == f"split={split}":
+            return float(f[3]) * points_per_launch
+    return None
+
+
 def cpu_baseline(system, budget_s=15.0):
     """reference (flang-built oracle/_ref/ref_driver) on a bounded sample"""
     from oracle import molio, refrun, sph
@@ -337,7 +352,12 @@ def main():
     roof["kernel"] = {1: "contract_bf16x6_kernel", 2: "contract_mx_kernel",
                       3: "contract_mx2_kernel<2,2>", 4: "contract_mx2_kernel<3,2>",
                       5: "contract_mx2_kernel<2,4>", 6: "contract_mx3_kernel<2>",
-                      7: "contract_mx3_kernel<3>"}.get(variant, "?")
+                      7: "contract_mx3p_kernel"}.get(variant, "?")
+    roof["split"] = split
+    roof["traffic"] = measured_traffic(roof["kernel"], split, roof["flop_per_launch"] / (14.0 * nbf * nbf))
+    if roof["traffic"] is not None:
+        roof["traffic_note"] = ("L2-miss bytes per launch (Infinity Cache + HBM), rocprofv3 FETCH_SIZE/"
+                                "WRITE_SIZE calibrated on a 1 GiB copy; profiles/contract_mx2_traffic_cage60.md")
     log("timing fp64")
     ms_f64, wall_f64 = time_steps(ctx, d_pts, out, "fp64", args.steps, args.warmup)
     J_f64_sample = out[:: max(1, npts // 65536)].cpu().numpy()

@@ -248,6 +248,173 @@ __global__ __launch_bounds__(256, 1) void contract_mx3_kernel(ContractArgs a) {
     }
 }
 
+// Pipelined form (three LDS buffers).  B fragments are read two MFMA groups
+// ahead from a ring of three register sets; the K-step's synchronisation
+// sits after its second group (wait until this wavefront's copy of block
+// ks+1 has landed, then a bare barrier), so the last two groups of K-step ks
+// already read the first two groups of K-step ks+1 and no LDS latency is
+// exposed at K-step boundaries.  The copy of block ks+2 is issued after
+// that barrier, into the buffer that held block ks-1, which every wavefront
+// has finished reading by then.  All A loads and copies are issued on every
+// K-step (indices clamped at the end) so the count of outstanding loads is
+// the same on every iteration.
+__device__ __forceinline__ int grp_pb(int grp) { return 2 - grp % 3; }
+__device__ __forceinline__ int grp_h(int grp) { return grp / 3; }
+
+__global__ __launch_bounds__(256, 1) void contract_mx3p_kernel(ContractArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned char *bbuf = smem;
+    double *qs = (double *)(smem + 3 * kBufBytes);
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c = lane & 31, q = lane >> 5;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int groups = (a.rows / kRows + 7) >> 3;
+    const int per_set = groups * a.nsplit;
+    const int set = slot / per_set;
+    const int split = slot % a.nsplit;
+    const int rtile = ((slot % per_set) / a.nsplit) * 8 + xcd;
+    if (rtile >= a.rows / kRows) return;
+    const size_t row0 = (size_t)rtile * kRows;
+    const int lda = a.lda;
+    const int nks = lda / 32;
+    const size_t rows = (size_t)a.rows;
+    const size_t gplane = (size_t)a.gplane;
+    const size_t arow = 3 * (size_t)lda;
+    for (int i = tid; i < kRows * kNQ; i += 256) qs[i] = 0.0;
+
+    const __bf16 *Aw = a.as3 + (row0 + wave * 32 + c) * arow + q * 24;
+    const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kBlk;
+    const int nt0 = split * a.tiles_per_split;
+    const int nt1 = min(nt0 + a.tiles_per_split, a.ntiles);
+    // vmcnt(6): all but the six A loads issued after the last copy
+    constexpr int kWaitAllButA = 6 | (7 << 4) | (15 << 8);
+
+    for (int nt = nt0; nt < nt1; nt++) {
+        const __bf16 *Bnt = Bset + (size_t)nt * nks * kBlk;
+        v16f acc[7];
+#pragma unroll
+        for (int m = 0; m < 7; m++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[m][e] = 0.f;
+
+        stage_block(Bnt, bbuf, wave, lane);
+        stage_block(Bnt + (size_t)min(1, nks - 1) * kBlk, bbuf + kBufBytes, wave, lane);
+        v8bf an[2][3];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int p = 0; p < 3; p++) an[h][p] = *(const v8bf *)(Aw + 2 * h * 24 + p * 8);
+        __builtin_amdgcn_s_waitcnt(kWaitAll);
+        __syncthreads();
+        v8bf bf[3][7];
+        read_group(bf[0], (const __bf16 *)bbuf, grp_pb(0), grp_h(0), q, c);
+        read_group(bf[1], (const __bf16 *)bbuf, grp_pb(1), grp_h(1), q, c);
+
+        for (int ks = 0; ks < nks; ks++) {
+            v8bf as[2][3];
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int p = 0; p < 3; p++) as[h][p] = an[h][p];
+            const __bf16 *cur = (const __bf16 *)(bbuf + (size_t)(ks % 3) * kBufBytes);
+            const __bf16 *nxt = (const __bf16 *)(bbuf + (size_t)((ks + 1) % 3) * kBufBytes);
+            const __bf16 *cblk = Bnt + (size_t)min(ks + 2, nks - 1) * kBlk;
+            unsigned char *cbuf = bbuf + (size_t)((ks + 2) % 3) * kBufBytes;
+            const size_t ka = (size_t)min(ks + 1, nks - 1);
+#pragma unroll
+            for (int grp = 0; grp < 6; grp++) {
+                // fragments two groups ahead (groups 0 and 1 of the next
+                // K-step from the next buffer, which the barrier after group
+                // 1 made complete)
+                if (grp + 2 < 6)
+                    read_group(bf[(grp + 2) % 3], cur, grp_pb(grp + 2), grp_h(grp + 2), q, c);
+                else
+                    read_group(bf[(grp + 2) % 3], nxt, grp_pb(grp - 4), grp_h(grp - 4), q, c);
+                if (grp < 2) {
+#pragma unroll
+                    for (int p = 0; p < 3; p++)
+                        an[grp][p] = *(const v8bf *)(Aw + (ka * 4 + 2 * grp) * 24 + p * 8);
+                } else {
+                    // pieces 0-2, 3-5, 6-8, 9-10 in groups 2..5
+#pragma unroll
+                    for (int i = 3 * (grp - 2); i < 3 * (grp - 2) + 3; i++)
+                        if (i < kPiecesPerWave) stage_piece(cblk, cbuf, wave, lane, i);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                const int h = grp_h(grp), pb = grp_pb(grp);
+#pragma unroll
+                for (int pa = 2 - pb; pa >= 0; pa--)
+#pragma unroll
+                    for (int m = 0; m < 7; m++)
+                        acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as[h][pa], bf[grp % 3][m],
+                                                                         acc[m], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (grp == 1) {
+                    // the copy of block ks+1 (issued during K-step ks-1) has
+                    // landed for this wavefront; after the barrier, for all
+                    __builtin_amdgcn_s_waitcnt(kWaitAllButA);
+                    __builtin_amdgcn_s_barrier();
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            // the next K-step's A fragments (issued before this K-step's
+            // copy) have landed; the copy may still be in flight.  The
+            // scheduling barriers keep the next K-step's register copies of
+            // those fragments behind this wait (ahead of it the compiler
+            // would wait for every outstanding load, the copy included).
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(kWaitAllButBlock);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // no copy may still be writing a buffer that the next N-tile reuses
+        __builtin_amdgcn_s_waitcnt(kWaitAll);
+        __builtin_amdgcn_s_barrier();
+
+        const int col = nt * kNu + c;
+        const float R[3] = {a.rf32[col], a.rf32[(size_t)a.ncol_pad + col],
+                            a.rf32[2 * (size_t)a.ncol_pad + col]};
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int rl = wave * 32 + 8 * (e >> 2) + 4 * q + (e & 3);
+            const size_t p = row0 + rl;
+            const bool valid = p < (size_t)a.npts;
+            const float rp[3] = {valid ? (float)a.pts[3 * p] : 0.f,
+                                 valid ? (float)a.pts[3 * p + 1] : 0.f,
+                                 valid ? (float)a.pts[3 * p + 2] : 0.f};
+            const __bf16 *pe = a.as3 + p * arow + (col >> 3) * 24 + (col & 7);
+            const float ph = ((float)pe[0] + (float)pe[8]) + (float)pe[16];
+            const size_t ei = p * lda + col;
+            const float G[3] = {a.g32[ei], a.g32[gplane + ei], a.g32[2 * gplane + ei]};
+            float v[32];
+            const float t0 = acc[0][e] * ph;
+            v[0] = t0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) v[1 + d] = t0 * (rp[d] - R[d]);
+#pragma unroll
+            for (int bb = 0; bb < 3; bb++)
+#pragma unroll
+                for (int m = 0; m < 3; m++) v[4 + m + 3 * bb] = acc[1 + bb][e] * G[m];
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int m = 0; m < 3; m++) v[13 + m + 3 * d] = G[m] * acc[4 + d][e];
+#pragma unroll
+            for (int i = kNQ; i < 32; i++) v[i] = 0.f;
+            const float sum = butterfly32(v, c);
+            if (c < kNQ) qs[rl * kNQ + c] += (double)sum;
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    __syncthreads();
+    double *out = a.part + ((size_t)set * a.nsplit + split) * kNQ * rows + row0;
+    for (int i = tid; i < kRows * kNQ; i += 256) {
+        const int qq = i / kRows, rl = i % kRows;
+        out[(size_t)qq * rows + rl] = qs[rl * kNQ + qq];
+    }
+}
+
 template <int NBUF>
 static hipError_t launch_mx3(const ContractArgs &a, hipStream_t s) {
     static bool attr = false;
@@ -267,9 +434,28 @@ static hipError_t launch_mx3(const ContractArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+static hipError_t launch_mx3p(const ContractArgs &a, hipStream_t s) {
+    static bool attr = false;
+    const size_t lds = 3 * kBufBytes + kLdsQ;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)contract_mx3p_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    if (a.rows % kRows || a.lda % 32 || a.ncol_pad != a.lda || a.ntiles * kNu != a.ncol_pad)
+        return hipErrorInvalidValue;
+    const long long groups = (a.rows / kRows + 7) / 8;
+    const long long nwg = 8 * groups * a.nsplit * a.nmatset;
+    if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(contract_mx3p_kernel, dim3((unsigned)nwg), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+// nbuf 2: double-buffered; 3: the pipelined three-buffer form
 hipError_t launch_contract_mx3(const ContractArgs &a, int nbuf, hipStream_t s) {
     if (nbuf == 2) return launch_mx3<2>(a, s);
-    if (nbuf == 3) return launch_mx3<3>(a, s);
+    if (nbuf == 3) return launch_mx3p(a, s);
     return hipErrorInvalidValue;
 }
 
