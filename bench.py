@@ -72,8 +72,17 @@ def time_steps(ctx, d_pts, out, precision, steps, warmup):
 
 
 def kernel_roofline(ctx, d_pts, out, precision, nbf):
-    """one extra pass with event timing around each kernel launch"""
+    """Two extra passes: one that counts the screening block lists of every
+    row tile (a tile of R rows with n listed 32-function blocks multiplies
+    R x 32n by 32n x 7*32n: 14 R (32 n)^2 flops), and one with event timing
+    around each kernel launch.  achieved = listed-block flops per launch /
+    average launch time; the dense product would need 14 nbf^2 per point."""
     import torch
+    ctx.set_tile_stats(True)
+    ctx.jtensor_device(d_pts, precision=precision, out=out)
+    torch.cuda.synchronize()
+    ts = ctx.tile_stats()
+    ctx.set_tile_stats(False)
     ctx.set_timing(True)
     ctx.jtensor_device(d_pts, precision=precision, out=out)
     torch.cuda.synchronize()
@@ -81,14 +90,21 @@ def kernel_roofline(ctx, d_pts, out, precision, nbf):
     ctx.set_timing(False)
     ms, launches = t["contract"]
     npts = d_pts.shape[0]
-    flop = 14.0 * nbf * nbf * npts  # algorithmic: 7 products of n x n per point
-    achieved = flop / (ms / launches) / launches / 1e9  # TFLOP/s per launch average
+    dense = 14.0 * nbf * nbf * npts
+    if ts["tiles"] > 0:
+        flop = 14.0 * ts["tile_rows"] * 1024.0 * ts["blocks_sq"]
+        block_pairs = ts["blocks_sq"] / (ts["tiles"] * ts["nblk"] ** 2)
+    else:  # a dense kernel
+        flop, block_pairs = dense, 1.0
     avg_ms = ms / launches
+    achieved = flop / launches / avg_ms / 1e9  # TFLOP/s per launch average
     return dict(bound="mfma", achieved=round(achieved, 2), peak=PEAK_TFLOPS[precision],
                 unit="TFLOP/s", frac=round(achieved / PEAK_TFLOPS[precision], 4), traffic=None,
                 kernel="contract_bf16x6_kernel" if precision != "fp64" else "contract_f64_kernel",
                 avg_launch_ms=round(avg_ms, 3), launches=launches,
-                flop_per_launch=flop / launches), t
+                flop_per_launch=flop / launches, dense_flop_per_launch=dense / launches,
+                listed_block_pairs=round(block_pairs, 4),
+                points_per_launch=npts / launches), t
 
 
 def measured_traffic(kernel, split, points_per_launch):
@@ -354,7 +370,7 @@ def main():
                       5: "contract_mx2_kernel<2,4>", 6: "contract_mx3_kernel<2>",
                       7: "contract_mx3p_kernel"}.get(variant, "?")
     roof["split"] = split
-    roof["traffic"] = measured_traffic(roof["kernel"], split, roof["flop_per_launch"] / (14.0 * nbf * nbf))
+    roof["traffic"] = measured_traffic(roof["kernel"], split, roof["points_per_launch"])
     if roof["traffic"] is not None:
         roof["traffic_note"] = ("L2-miss bytes per launch (Infinity Cache + HBM), rocprofv3 FETCH_SIZE/"
                                 "WRITE_SIZE calibrated on a 1 GiB copy; profiles/contract_mx2_traffic_cage60.md")

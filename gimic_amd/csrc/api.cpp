@@ -111,17 +111,30 @@ struct gimic_ctx {
     // workspace
     DevBuf phi64, grad64, phi32, grad32, phis, part, pts_tmp, jt_tmp, contrib, planes,
         pparams, offs, outs;
+    // internal function order (see setup_device): fun_orig[i] is the caller's
+    // index of internal function i; per 32-function block the shells with a
+    // function in it (blk_beg ranges into blk_sh)
+    std::vector<int> fun_orig;
+    DevBuf blk_beg, blk_sh;
+    // spatial tiles of a chunk: sorted points, their caller indices, sort
+    // scratch, and the screening block list of every contraction row tile
+    DevBuf spts, perm, bpart, sortscr, bcount, blist;
     // optional per-kernel timing with HIP events on the launch stream:
     // [0] basis evaluation, [1] contraction, [2] finalize
     bool timing = false;
     // mixed-precision contraction kernel chosen by timing (-1: not yet tuned;
     // 1 register-tiled, 2 LDS-tiled, 3 and 4 pre-split operands with
     // direct-to-LDS staging, double or triple buffered, 5 pre-split with 256
-    // rows per workgroup, 6 and 7 pre-split on 32x32 MFMA blocks, double or
-    // triple buffered) and the timings (basis evaluation + contraction) that
-    // chose it
+    // rows per workgroup, 6 and 7 pre-split on 32x32 MFMA blocks, double
+    // buffered or pipelined) and the timings (basis evaluation +
+    // contraction) that chose it.  Variants 0 (FP64) and 3-5 walk the
+    // screening block lists; 1, 2, 6 and 7 compute the dense product and are
+    // not among the tuner's candidates (set_mixed_kernel selects them)
     int mixed_variant = -1;
     int mixed_split = 0;  // N-split count of the pre-split kernel (0: by workgroup count)
+    // block-list statistics (gimic_ctx_set_tile_stats): tiles, sum n, sum n^2, rows per tile
+    bool tile_stats = false;
+    double tile_acc[4] = {0.0, 0.0, 0.0, 0.0};
     float tune_ms[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     std::vector<std::pair<int, float>> tune_log;  // (variant * 100 + split, ms)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
@@ -158,10 +171,73 @@ void setup_device(gimic_ctx *c) {
     finalize_basis(b, c->screen_thrs);
     c->lda = (int)round_up((size_t)b.nbf, kKPad);
     c->ncol_pad = (int)round_up((size_t)b.nbf, 32);
+    // Internal function order: shells by decreasing screening radius, then
+    // by atom along a nearest-neighbour path through the molecule, so that a
+    // 32-function block holds functions of like range centred close
+    // together and screening zeroes whole blocks over a compact tile of
+    // points.  Within a shell the order is the caller's.  J is a sum over
+    // function pairs, so only the summation order changes.
+    const int nsh = (int)b.shells.size(), nat = b.natoms();
+    std::vector<int> arank(std::max(nat, 1), 0);
+    {
+        std::vector<char> used(nat, 0);
+        int cur = 0;
+        for (int k = 0; k < nat; k++) {
+            arank[cur] = k;
+            used[cur] = 1;
+            int best = -1;
+            double bd = 0.0;
+            for (int j = 0; j < nat; j++) {
+                if (used[j]) continue;
+                double d2 = 0.0;
+                for (int d = 0; d < 3; d++) {
+                    const double e = b.coords[3 * j + d] - b.coords[3 * cur + d];
+                    d2 += e * e;
+                }
+                if (best < 0 || d2 < bd) {
+                    best = j;
+                    bd = d2;
+                }
+            }
+            if (best < 0) break;
+            cur = best;
+        }
+    }
+    std::vector<int> order(nsh);
+    for (int i = 0; i < nsh; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const Shell &p = b.shells[x], &q = b.shells[y];
+        if (p.radius != q.radius) return p.radius > q.radius;
+        return arank[p.atom] < arank[q.atom];
+    });
+    auto nout_of = [&](const Shell &s) { return b.spherical ? 2 * s.l + 1 : ncart(s.l); };
+    std::vector<int> fnew(nsh, 0);
+    c->fun_orig.assign(b.nbf, 0);
+    {
+        int pos = 0;
+        for (int si : order) {
+            const Shell &s = b.shells[si];
+            fnew[si] = pos;
+            for (int f = 0; f < nout_of(s); f++) c->fun_orig[pos + f] = s.fout + f;
+            pos += nout_of(s);
+        }
+    }
+    const int nblk = c->lda / 32;
+    std::vector<int> blk_beg(nblk + 1, 0), blk_sh;
+    for (int k = 0; k < nblk; k++) {
+        blk_beg[k] = (int)blk_sh.size();
+        for (int si : order) {
+            const int f0 = fnew[si], f1 = f0 + nout_of(b.shells[si]);
+            if (f0 < 32 * (k + 1) && f1 > 32 * k) blk_sh.push_back(si);
+        }
+    }
+    blk_beg[nblk] = (int)blk_sh.size();
+    if (blk_sh.empty()) blk_sh.push_back(0);
     std::vector<DevShell> sh;
-    for (auto &s : b.shells)
-        sh.push_back(DevShell{s.atom, s.l, s.nprim, s.poff, s.fout,
-                              b.spherical ? 2 * s.l + 1 : ncart(s.l), s.radius});
+    for (int i = 0; i < nsh; i++) {
+        const Shell &s = b.shells[i];
+        sh.push_back(DevShell{s.atom, s.l, s.nprim, s.poff, fnew[i], nout_of(s), s.radius});
+    }
     hipStream_t st = c->stream;
     auto up = [&](DevBuf &d, const void *src, size_t n) {
         d.alloc(n, st, false);
@@ -171,14 +247,16 @@ void setup_device(gimic_ctx *c) {
     up(c->exps, b.exps.data(), b.exps.size() * sizeof(double));
     up(c->ncc, b.ncc.data(), b.ncc.size() * sizeof(double));
     up(c->coords, b.coords.data(), b.coords.size() * sizeof(double));
+    up(c->blk_beg, blk_beg.data(), blk_beg.size() * sizeof(int));
+    up(c->blk_sh, blk_sh.data(), blk_sh.size() * sizeof(int));
     std::vector<double> rf(3 * (size_t)c->ncol_pad, 0.0);
     std::vector<float> rf32(3 * (size_t)c->ncol_pad, 0.f);
-    for (auto &s : b.shells) {
-        const int nout = b.spherical ? 2 * s.l + 1 : ncart(s.l);
-        for (int f = 0; f < nout; f++)
+    for (int i = 0; i < nsh; i++) {
+        const Shell &s = b.shells[i];
+        for (int f = 0; f < nout_of(s); f++)
             for (int d = 0; d < 3; d++) {
-                rf[(size_t)d * c->ncol_pad + s.fout + f] = b.coords[3 * s.atom + d];
-                rf32[(size_t)d * c->ncol_pad + s.fout + f] = (float)b.coords[3 * s.atom + d];
+                rf[(size_t)d * c->ncol_pad + fnew[i] + f] = b.coords[3 * s.atom + d];
+                rf32[(size_t)d * c->ncol_pad + fnew[i] + f] = (float)b.coords[3 * s.atom + d];
             }
     }
     up(c->rf64, rf.data(), rf.size() * sizeof(double));
@@ -225,26 +303,31 @@ gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec)
     auto m = std::make_unique<gimic_ctx::Mats>();
     m->nmatset = (int)spins.size();
     std::vector<double> B(setsz * spins.size(), 0.0);
+    // atom of each caller-ordered function; the internal order is fun_orig
     std::vector<int> atom_of(n);
     for (auto &s : c->basis.shells) {
         const int nout = c->basis.spherical ? 2 * s.l + 1 : ncart(s.l);
         for (int f = 0; f < nout; f++) atom_of[s.fout + f] = s.atom;
     }
+    const std::vector<int> &fo = c->fun_orig;
     for (size_t si = 0; si < spins.size(); si++) {
         std::vector<double> fam = spin_family(c, spins[si]);
         double *Bs = &B[si * setsz];
-        // matrix j column nu (K = mu contiguous): D(mu,nu) = fam[blk + mu + n*nu].
+        // matrix j column nu (K = mu contiguous), internal order:
+        // D(mu,nu) = fam[blk + fo[mu] + n*fo[nu]].
         // The GIAO right operands are W_d(mu,nu) = D(mu,nu) (R_nu - R_mu)_d:
         // phi W_d equals R_d (phi D) - phi diag(R_d) D without the
         // cancellation between those two terms (and is zero on one-atom blocks).
         for (size_t nu = 0; nu < n; nu++) {
-            const double *Rn = &c->basis.coords[3 * atom_of[nu]];
+            const size_t onu = (size_t)fo[nu];
+            const double *Rn = &c->basis.coords[3 * atom_of[onu]];
             for (size_t mu = 0; mu < n; mu++) {
-                const double D = fam[mu + n * nu];
-                const double *Rm = &c->basis.coords[3 * atom_of[mu]];
+                const size_t omu = (size_t)fo[mu];
+                const double D = fam[omu + n * onu];
+                const double *Rm = &c->basis.coords[3 * atom_of[omu]];
                 Bs[0 * matsz + nu * ld + mu] = D;
                 for (int b = 1; b < 4; b++)
-                    Bs[b * matsz + nu * ld + mu] = fam[(size_t)b * n * n + mu + n * nu];
+                    Bs[b * matsz + nu * ld + mu] = fam[(size_t)b * n * n + omu + n * onu];
                 for (int d = 0; d < 3; d++)
                     Bs[(4 + d) * matsz + nu * ld + mu] = D * (Rn[d] - Rm[d]);
             }
@@ -395,13 +478,23 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         grow(c->phis, rows_cap * ld * 6, c->phis.bytes < rows_cap * ld * 6);
         grow(c->grad32, 3 * rows_cap * ld * 4, c->grad32.bytes < 3 * rows_cap * ld * 4);
     }
+    // spatial tiles: sorted points, caller indices, sort scratch, and block
+    // lists for row tiles of 128 rows or more
+    const int nblk = c->lda / 32;
+    grow(c->spts, rows_cap * 3 * 8, false);
+    grow(c->perm, rows_cap * 4, false);
+    grow(c->bpart, 256 * 6 * 8, false);
+    grow(c->sortscr, (size_t)sort_scratch_ints((int)rows_cap) * 4, false);
+    grow(c->bcount, rows_cap / 128 * 4, false);
+    grow(c->blist, rows_cap / 128 * nblk * 4, false);
     const int nmatset = M.nmatset;
     for (size_t start = 0; start < (size_t)npts; start += chunk) {
         const int n_c = (int)std::min(chunk, (size_t)npts - start);
         const int rows = (int)round_up((size_t)n_c, gran);
 
+        const double *spts = c->spts.as<double>();
         EvalArgs ea{};
-        ea.pts = d_r + 3 * start;
+        ea.pts = spts;
         ea.npts = n_c;
         ea.rows = rows;
         ea.gplane = gplane;
@@ -427,7 +520,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         ca.ncol_pad = c->ncol_pad;
         ca.gplane = gplane;
         ca.nmatset = nmatset;
-        ca.pts = d_r + 3 * start;
+        ca.pts = spts;
         ca.npts = n_c;
         ca.a64 = c->phi64.as<double>();
         ca.g64 = c->grad64.as<double>();
@@ -439,6 +532,39 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         ca.bs3 = M.bs3.as<__bf16>();
         ca.rf64 = c->rf64.as<double>();
         ca.rf32 = c->rf32.as<float>();
+        ca.bcount = c->bcount.as<int>();
+        ca.blist = c->blist.as<int>();
+        ca.nblk = nblk;
+        // screening block lists for row tiles of tile_rows rows
+        auto lists = [&](int tile_rows) {
+            TileArgs ta{};
+            ta.spts = spts;
+            ta.npts = n_c;
+            ta.rows = rows;
+            ta.tile_rows = tile_rows;
+            ta.nblk = nblk;
+            ta.blk_beg = c->blk_beg.as<int>();
+            ta.blk_sh = c->blk_sh.as<int>();
+            ta.shells = c->shells.as<DevShell>();
+            ta.coords = c->coords.as<double>();
+            ta.bcount = c->bcount.as<int>();
+            ta.blist = c->blist.as<int>();
+            hip_check(launch_block_lists(ta, st), "block_lists");
+            if (c->tile_stats && mode == EvalMode::Eager) {
+                const int nt = rows / tile_rows;
+                std::vector<int> cnt(nt);
+                hip_check(hipMemcpyAsync(cnt.data(), ta.bcount, nt * sizeof(int),
+                                         hipMemcpyDeviceToHost, st),
+                          "hipMemcpyAsync");
+                hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+                for (int k : cnt) {
+                    c->tile_acc[0] += 1.0;
+                    c->tile_acc[1] += k;
+                    c->tile_acc[2] += (double)k * k;
+                }
+                c->tile_acc[3] = tile_rows;
+            }
+        };
         // work split over N so a launch has at least ~2048 workgroups, or
         // the split count the tuner chose for the pre-split kernel
         auto plan = [&](int wg_rows, int tile_cols, int want_split = 0) {
@@ -461,6 +587,9 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         };
         auto run = [&](int variant, int split = -1) {
             plan_variant(variant, split);
+            // kernels that walk block lists: FP64 and pre-split 16x16 (128 or 256 rows)
+            if (variant == 0 || (variant >= 3 && variant <= 5))
+                lists(variant == 0 ? kWgRows : variant == 5 ? 256 : contract_mx_rows());
             if (variant == 0) hip_check(launch_contract_f64(ca, st), "contract_f64");
             else if (variant == 1) hip_check(launch_contract_bf16x6(ca, st), "contract_bf16x6");
             else if (variant == 2) hip_check(launch_contract_mx(ca, st), "contract_mx");
@@ -480,16 +609,25 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             plan_variant(prec == GIMIC_PREC_FP64 ? 0 : c->mixed_variant);
             continue;
         }
+        // rows in Morton order of their cells (spts, perm)
+        timed(c, 0, st, [&] {
+            hip_check(launch_sort_points(d_r + 3 * start, n_c, c->spts.as<double>(),
+                                         c->perm.as<int>(), c->bpart.as<double>(),
+                                         c->sortscr.as<int>(), st),
+                      "sort_points");
+        });
         if (tune) {
             // autotune once per context on this chunk: basis evaluation +
-            // contraction for every mixed kernel, and for the pre-split
-            // kernel every N-split count (more splits give more workgroups
-            // that share each row tile's A through L2); every candidate is
-            // timed twice and its faster run counts; keep the fastest
+            // contraction for the pre-split kernels that walk block lists,
+            // each with several N-split counts (more splits give more
+            // workgroups that share each row tile's A through L2, fewer
+            // give each workgroup more N-blocks to amortise its start);
+            // every candidate is timed twice and its faster run counts; keep
+            // the fastest
             float best = 0.f;
             int best_split = 0;
-            const int cand[][2] = {{3, 16}, {3, 64}, {4, 16}, {4, 64}, {5, 16}, {5, 64},
-                                   {6, 8}, {6, 16}, {6, 64}, {7, 8}, {7, 16}, {7, 64}};
+            const int cand[][2] = {{3, 4}, {3, 8}, {3, 16}, {4, 4}, {4, 8}, {4, 16},
+                                   {5, 4}, {5, 8}, {5, 16}};
             c->tune_log.clear();
             for (auto &cv : cand) {
                 const int v = cv[0], sp = cv[1];
@@ -535,7 +673,8 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         fa.rows = rows;
         fa.npts = n_c;
         fa.nmatset = nmatset;
-        fa.pts = d_r + 3 * start;
+        fa.pts = spts;
+        fa.perm = c->perm.as<int>();
         fa.giao = c->giao;
         fa.diamag = c->diamag;
         fa.paramag = c->paramag;
@@ -811,7 +950,10 @@ int gimic_flux_planes(gimic_ctx *c, int nplanes, const int64_t *offs, const doub
                 (uintptr_t)c->pparams.p, (uintptr_t)c->offs.p, (uintptr_t)c->outs.p,
                 (uintptr_t)c->phi64.p, (uintptr_t)c->grad64.p, (uintptr_t)c->phi32.p,
                 (uintptr_t)c->grad32.p, (uintptr_t)c->part.p, (uintptr_t)M.b64.p,
-                (uintptr_t)M.b32.p};
+                (uintptr_t)M.b32.p, (uintptr_t)c->phis.p, (uintptr_t)M.bs3.p,
+                (uintptr_t)c->mixed_split, (uintptr_t)c->spts.p, (uintptr_t)c->perm.p,
+                (uintptr_t)c->bpart.p, (uintptr_t)c->sortscr.p, (uintptr_t)c->bcount.p,
+                (uintptr_t)c->blist.p};
             if (!c->graph_exec || key != c->graph_key) {
                 if (c->graph_exec) {
                     hip_check(hipGraphExecDestroy(c->graph_exec), "hipGraphExecDestroy");
@@ -873,6 +1015,23 @@ int gimic_ctx_set_mixed_kernel(gimic_ctx *c, int variant, int split) {
         c->mixed_variant = variant == 0 ? -1 : variant;
         c->mixed_split = variant >= 3 ? split : 0;
         for (auto &e : c->tune_ms) e = 0.f;
+    });
+}
+
+int gimic_ctx_set_tile_stats(gimic_ctx *c, int on) {
+    return guarded([&] {
+        if (!c) throw Error(GIMIC_EINVAL, "null context");
+        c->tile_stats = on != 0;
+        if (c->tile_stats)
+            for (auto &v : c->tile_acc) v = 0.0;
+    });
+}
+
+int gimic_ctx_tile_stats(const gimic_ctx *c, double *out) {
+    return guarded([&] {
+        if (!c || !out) throw Error(GIMIC_EINVAL, "null argument");
+        for (int k = 0; k < 4; k++) out[k] = c->tile_acc[k];
+        out[4] = c->lda / 32;
     });
 }
 

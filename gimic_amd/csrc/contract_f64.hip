@@ -18,6 +18,10 @@
 // Tiling: 256 threads = 4 wavefronts; a workgroup owns 128 point rows
 // (32 per wavefront: two 16-row MFMA blocks) and walks a range of 16-column
 // N-tiles; per N-tile each wavefront keeps 2 x 7 v_mfma_f64_16x16x4 blocks.
+// Only the 32-function blocks listed for the row tile (those that screening
+// does not zero over it, tiles.hip) are visited, along K and along N: the
+// N-tiles are the listed blocks' halves, split evenly over the workgroups
+// that share the row tile.
 // The K index is remapped inside every 16-wide K-step so each lane loads 32
 // contiguous bytes of its A row and of each B column (lane group g holds
 // k = 4g..4g+3; sub-step s feeds k = 4g+s to both operands).
@@ -31,6 +35,8 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs a) {
     const int g = lane >> 4, c = lane & 15;
     const int set = blockIdx.z, split = blockIdx.y;
     const size_t row0 = (size_t)blockIdx.x * kWgRows;
+    const int *L = a.blist + (size_t)blockIdx.x * a.nblk;
+    const int nk = 2 * a.bcount[blockIdx.x];  // 16-wide K-steps (and N-tiles) listed
     const int lda = a.lda;
     const size_t rows = (size_t)a.rows;
     const size_t matsz = (size_t)a.ncol_pad * lda;
@@ -41,23 +47,30 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs a) {
     const double *A0 = a.a64 + (row0 + wave * 32 + c) * lda + 4 * g;
     const double *A1 = A0 + 16 * (size_t)lda;
     const double *Bset = a.b64 + (size_t)set * kNMat * matsz;
-    const int nt0 = split * a.tiles_per_split;
-    const int nt1 = min(nt0 + a.tiles_per_split, a.ntiles);
+    const int e0 = (int)((long long)split * nk / a.nsplit);
+    const int e1 = (int)((long long)(split + 1) * nk / a.nsplit);
+    // first element of listed K-step kk
+    auto kof = [&](int kk) { return L[kk >> 1] * 32 + (kk & 1) * 16; };
 
-    for (int nt = nt0; nt < nt1; nt++) {
+    for (int e = e0; e < e1; e++) {
+        const int nt = kof(e) / kNT;
         const int col = nt * kNT + c;
         const double *Bc = Bset + (size_t)col * lda + 4 * g;
         v4d acc[2][kNMat];
 #pragma unroll
         for (int m = 0; m < kNMat; m++) acc[0][m] = acc[1][m] = v4d{0.0, 0.0, 0.0, 0.0};
 
-        v4d a0 = *(const v4d *)(A0), a1 = *(const v4d *)(A1);
+        const int k0 = kof(0);
+        v4d a0 = *(const v4d *)(A0 + k0), a1 = *(const v4d *)(A1 + k0);
         v4d b[kNMat];
 #pragma unroll
-        for (int m = 0; m < kNMat; m++) b[m] = *(const v4d *)(Bc + m * matsz);
-        for (int k0 = 0; k0 < lda; k0 += 16) {
+        for (int m = 0; m < kNMat; m++) b[m] = *(const v4d *)(Bc + m * matsz + k0);
+        // element offset of the next listed K-step, read one step early
+        int knext = kof(nk > 1 ? 1 : 0);
+        for (int kk = 0; kk < nk; kk++) {
             // prefetch the next K-step while this one runs on the matrix cores
-            const int kn = k0 + 16 < lda ? k0 + 16 : k0;
+            const int kn = knext;
+            knext = kof(kk + 2 < nk ? kk + 2 : nk - 1);
             const v4d na0 = *(const v4d *)(A0 + kn), na1 = *(const v4d *)(A1 + kn);
             v4d nb[kNMat];
 #pragma unroll

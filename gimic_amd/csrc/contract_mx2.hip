@@ -24,6 +24,12 @@
 // (half the LDS traffic per MFMA of KRB = 2) and doubles the work per K-step
 // that hides the next step's loads; it needs 224 accumulator registers.
 //
+// Screening.  A workgroup visits only the 32-function blocks listed for its
+// row tile (tiles.hip): the K-loop walks the list, and the listed N-blocks
+// are split evenly over the nsplit workgroups of the row tile.  The LDS
+// buffers rotate over list positions, so the pipeline is the same as for a
+// dense walk.
+//
 // Workgroup order.  Every workgroup re-reads its 128 rows of A once per
 // N-tile, so A traffic is what limits this kernel.  The grid is 1-D and
 // decoded so that the nsplit workgroups that share a row tile (one per
@@ -115,10 +121,13 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
     // lane (g, c) of row block rb: row wave*32 + rb*16 + c, k = 32 ks + 8 g .. + 7
     const __bf16 *Aw = a.as3 + (row0 + wave * 16 * kRB + c) * arow + g * 24;
     const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kBlk;
-    const int nt0 = split * a.tiles_per_split;
-    const int nt1 = min(nt0 + a.tiles_per_split, a.ntiles);
+    const int *L = a.blist + (size_t)rtile * a.nblk;
+    const int na = a.bcount[rtile];  // listed blocks
+    const int e0 = (int)((long long)split * na / a.nsplit);
+    const int e1 = (int)((long long)(split + 1) * na / a.nsplit);
 
-    for (int nt = nt0; nt < nt1; nt++) {
+    for (int e = e0; e < e1; e++) {
+        const int nt = L[e];
         const __bf16 *Bnt = Bset + (size_t)nt * nks * kBlk;
         v4f acc[kRB][kCB];
 #pragma unroll
@@ -126,25 +135,33 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
 #pragma unroll
             for (int cb = 0; cb < kCB; cb++) acc[rb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
 
-        stage_block(Bnt, bbuf, wave, lane);
-        if (NBUF == 3 && nks > 1) stage_block(Bnt + kBlk, bbuf + kBufBytes, wave, lane);
+        stage_block(Bnt + (size_t)L[0] * kBlk, bbuf, wave, lane);
+        if (NBUF == 3 && na > 1) stage_block(Bnt + (size_t)L[1] * kBlk, bbuf + kBufBytes, wave, lane);
         v8bf an[kRB][3];
 #pragma unroll
         for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
-            for (int p = 0; p < 3; p++) an[rb][p] = *(const v8bf *)(Aw + rb * 16 * arow + p * 8);
+            for (int p = 0; p < 3; p++)
+                an[rb][p] = *(const v8bf *)(Aw + rb * 16 * arow + (size_t)L[0] * 96 + p * 8);
         __builtin_amdgcn_s_waitcnt(kWaitAll);
         __syncthreads();
 
-        for (int ks = 0; ks < nks; ks++) {
+        // blocks of the next step's A fragments and of the next step's copy,
+        // read one K-step early (the end-of-step wait covers the scalar loads)
+        int blk_a = L[na > 1 ? 1 : 0];
+        int blk_c = L[min(NBUF - 1, na - 1)];
+        for (int ks = 0; ks < na; ks++) {
+            const int a_blk = blk_a, c_blk = blk_c;
+            blk_a = L[min(ks + 2, na - 1)];
+            blk_c = L[min(ks + NBUF, na - 1)];
             v8bf as[kRB][3];
 #pragma unroll
             for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
                 for (int p = 0; p < 3; p++) as[rb][p] = an[rb][p];
             // Block streamed during this K-step.  With NBUF = 3 the copy is
-            // issued every K-step (the last two repeat block nks-1 into a
-            // buffer nobody reads again) so that the count of outstanding
+            // issued every K-step (the last two repeat the last listed block
+            // into a buffer nobody reads again) so that the count of outstanding
             // loads is the same on every iteration.
             // Vector-memory instructions (the next K-step's A fragments, two
             // per column block from the first one on, and the block copy's 11
@@ -154,13 +171,13 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
             // works through them.  With NBUF = 3 the copy comes after all A
             // loads (column blocks 3 to 13), so that waiting for "all but the
             // last copy" covers the A loads.
-            const bool more = ks + 1 < nks;
+            const bool more = ks + 1 < na;
             constexpr int kALoads = 3 * kRB;
             constexpr int kCopyFrom = NBUF == 3 ? kCB - kPiecesPerWave : 0;
             static_assert(NBUF != 3 || (kALoads + 1) / 2 <= kCopyFrom, "A loads must precede the copy");
             const int ahead = ks + NBUF - 1;
-            const bool copy = NBUF == 3 || ahead < nks;
-            const __bf16 *cblk = Bnt + (size_t)min(ahead, nks - 1) * kBlk;
+            const bool copy = NBUF == 3 || ahead < na;
+            const __bf16 *cblk = Bnt + (size_t)c_blk * kBlk;
             unsigned char *cbuf = bbuf + (size_t)(ahead % NBUF) * kBufBytes;
             const __bf16 *cur = (const __bf16 *)(bbuf + (size_t)(ks % NBUF) * kBufBytes);
             v8bf bf[2][3];
@@ -182,7 +199,7 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
                 for (int j = 2 * cb; j < 2 * cb + 2; j++)
                     if (more && j < kALoads)
                         an[j / 3][j % 3] = *(const v8bf *)(Aw + (j / 3) * 16 * arow +
-                                                          (size_t)(ks + 1) * 96 + (j % 3) * 8);
+                                                          (size_t)a_blk * 96 + (j % 3) * 8);
                 if (copy && cb >= kCopyFrom && cb - kCopyFrom < kPiecesPerWave)
                     stage_piece(cblk, cbuf, wave, lane, cb - kCopyFrom);
                 __builtin_amdgcn_sched_barrier(0);

@@ -88,6 +88,11 @@ struct ContractArgs {
     const double *rf64;  // [3][ncol_pad] atom coordinate of each function
     const float *rf32;
     double *part;        // [nmatset][nsplit][kNQ][rows]
+    // screening block lists per row tile (kernels that walk them): bcount[t]
+    // 32-function blocks, listed ascending in blist[t * nblk ...]; split s of
+    // a tile takes the listed N-blocks [s * count / nsplit, (s+1) * count / nsplit)
+    const int *bcount, *blist;
+    int nblk;
 };
 hipError_t launch_contract_f64(const ContractArgs &a, hipStream_t s);
 hipError_t launch_contract_bf16x6(const ContractArgs &a, hipStream_t s);
@@ -107,7 +112,8 @@ constexpr int kMx2BlockElems = 22528;                 // 3*4*224*8 = 21504, padd
 struct FinalizeArgs {
     const double *part;  // [nmatset][nsplit][kNQ][rows]
     int nsplit, rows, npts, nmatset;
-    const double *pts;   // chunk points
+    const double *pts;   // chunk points in row order
+    const int *perm;     // caller index (within the chunk) of each row, or null
     int giao, diamag, paramag;
     double *jt;          // [nmatset][npts][9] (chunk offset applied)
     long long jt_stride; // elements between matrix sets in jt
@@ -124,6 +130,26 @@ hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s);
 // deterministic per-plane sums of contributions: out[3*plane] = net, pos, neg
 hipError_t launch_plane_reduce(const double *contrib, const long long *offs, int nplanes,
                                double *out, hipStream_t s);
+
+// Spatial tiles (tiles.hip).  launch_sort_points orders n points along a
+// Morton curve of 32^3 cells over their bounding box (stable, so the same on
+// every run): spts[j] = pts[perm[j]]; bpart holds 256 * 6 doubles, scratch
+// sort_scratch_ints(n) ints.
+int sort_scratch_ints(int n);
+hipError_t launch_sort_points(const double *pts, int n, double *spts, int *perm, double *bpart,
+                              int *scratch, hipStream_t s);
+struct TileArgs {
+    const double *spts;  // sorted chunk points
+    int npts, rows;      // valid and padded rows
+    int tile_rows;       // rows per contraction workgroup
+    int nblk;            // 32-function blocks (lda / 32)
+    const int *blk_beg;  // [nblk + 1] ranges into blk_sh
+    const int *blk_sh;   // shells with a function in each block
+    const DevShell *shells;
+    const double *coords;
+    int *bcount, *blist;  // [rows / tile_rows], [rows / tile_rows][nblk]
+};
+hipError_t launch_block_lists(const TileArgs &a, hipStream_t s);
 
 hipError_t run_mfma_layout_probe(double *h_f64, float *h_bf16);
 hipError_t run_mfma32_layout_probe(float *h_out);  // 2048 floats

@@ -26,6 +26,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= a.npts) return;
     const double r[3] = {a.pts[3 * p], a.pts[3 * p + 1], a.pts[3 * p + 2]};
+    const int o = a.perm ? a.perm[p] : p;  // caller index of row p
     for (int set = 0; set < a.nmatset; set++) {
         double q[kNQ];
 #pragma unroll
@@ -83,11 +84,11 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
                 }
                 J[m + 3 * b] = v;
             }
-        double *o = a.jt + (size_t)set * a.jt_stride + (size_t)p * 9;
+        double *out = a.jt + (size_t)set * a.jt_stride + (size_t)o * 9;
 #pragma unroll
-        for (int i = 0; i < 9; i++) o[i] = J[i];
+        for (int i = 0; i < 9; i++) out[i] = J[i];
         if (set == 0 && a.contrib) {
-            const int pl = a.plane ? a.plane[p] : 0;
+            const int pl = a.plane ? a.plane[o] : 0;
             const double *n = a.pnorm + 3 * pl, *B = a.pbfield + 3 * pl, *ctr = a.pcenter + 3 * pl;
             double dot = 0.0;
 #pragma unroll
@@ -95,8 +96,8 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
                 dot += n[m] * (J[m] * B[0] + J[m + 3] * B[1] + J[m + 6] * B[2]);
             const double dx = r[0] - ctr[0], dy = r[1] - ctr[1], dz = r[2] - ctr[2];
             const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-            const bool out = dist > a.pradius[pl];
-            a.contrib[p] = out ? 0.0 : dot * a.w[p];
+            const bool outside = dist > a.pradius[pl];
+            a.contrib[o] = outside ? 0.0 : dot * a.w[o];
             if (a.contrib_mod) {
                 double jv2 = 0.0;
 #pragma unroll
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
                     jv2 += v * v;
                 }
                 const double sgn = fabs(dot) < 1e-12 ? 0.0 : (dot > 0.0 ? 1.0 : -1.0);
-                a.contrib_mod[p] = out ? 0.0 : sgn * sqrt(jv2) * a.w[p];
+                a.contrib_mod[o] = outside ? 0.0 : sgn * sqrt(jv2) * a.w[o];
             }
             if (a.contrib_acid) {
                 // get_acid, acid.f90:11-40 (DP33 = 0.3333333 as in the reference)
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
                 const double o1 = J[3] + J[1], o2 = J[6] + J[2], o3 = J[7] + J[5];
                 const double dt2 = 0.3333333 * (d1 * d1 + d2 * d2 + d3 * d3) +
                                    0.5 * (o1 * o1 + o2 * o2 + o3 * o3);
-                a.contrib_acid[p] = out ? 0.0 : dt2 * a.w[p];
+                a.contrib_acid[o] = outside ? 0.0 : dt2 * a.w[o];
             }
         }
     }

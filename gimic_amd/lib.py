@@ -19,7 +19,7 @@ EXPORTS = [
     "gimic_set_screening", "gimic_calc_jtensor", "gimic_calc_jvector", "gimic_calc_modj",
     "gimic_get_gauss_points", "mkgausspoints", "gimic_last_error", "gimic_ctx_create_files",
     "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
-    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_tuning_log", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
+    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_tuning_log", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_tile_stats", "gimic_ctx_tile_stats", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
     "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_sph_transform", "gimic_quadrature",
     "gimic_selftest_mfma_layout", "gimic_selftest_mfma32_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_set_stream", "gimic_mlp_info",
     "gimic_mlp_params", "gimic_mlp_loss_grad", "gimic_mlp_train_step", "gimic_mlp_predict",
@@ -77,6 +77,8 @@ def load():
         "gimic_ctx_tuning": [P, P, P, P],
         "gimic_ctx_tuning_log": [P, P, P, P],
         "gimic_ctx_set_mixed_kernel": [P, I, I],
+        "gimic_ctx_set_tile_stats": [P, I],
+        "gimic_ctx_tile_stats": [P, P],
         "gimic_ctx_set_graph": [P, I],
         "gimic_ctx_graph_stats": [P, P, P],
         "gimic_jtensor_batch": [P, I64, P, P, I, I],
@@ -234,10 +236,25 @@ class Context:
     def set_mixed_kernel(self, variant=0, split=0):
         """fix the mixed contraction kernel (1 register-tiled, 2 LDS-tiled,
         3/4 pre-split double/triple buffered, 5 pre-split with 256-row
-        tiles, 6/7 pre-split on 32x32 MFMA blocks double/triple buffered,
+        tiles, 6/7 pre-split on 32x32 MFMA blocks double buffered/pipelined;
+        3-5 visit only the blocks screening leaves non-zero,
         with `split` N-ranges, 0 = by workgroup count) or 0 for automatic
         selection"""
         check(self._lib.gimic_ctx_set_mixed_kernel(self._h, int(variant), int(split)))
+
+    def set_tile_stats(self, on=True):
+        """collect screening block-list counts of eager evaluations (each
+        copies its counts to the host: keep it off in timed runs)"""
+        check(self._lib.gimic_ctx_set_tile_stats(self._h, 1 if on else 0))
+
+    def tile_stats(self):
+        """dict(tiles, blocks, blocks_sq, tile_rows, nblk): row tiles seen,
+        sum of listed blocks and of their squares, rows per tile, blocks of
+        a dense row"""
+        out = np.zeros(5)
+        check(self._lib.gimic_ctx_tile_stats(self._h, ptr(out)))
+        return dict(tiles=int(out[0]), blocks=float(out[1]), blocks_sq=float(out[2]),
+                    tile_rows=int(out[3]), nblk=int(out[4]))
 
     def set_graph(self, enable=True):
         """capture flux_planes batches into one reusable hipGraph launch"""

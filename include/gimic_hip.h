@@ -128,12 +128,14 @@ int gimic_ctx_get_timing(gimic_ctx *ctx, double *ms, long long *launches);
  * Kernels: register-tiled (1), LDS-tiled (2), pre-split operands on 16x16
  * MFMA blocks double (3) or triple (4) buffered with 128 rows per workgroup
  * or double buffered with 256 rows (5), pre-split on 32x32 MFMA blocks
- * double (6) or triple (7) buffered.  The first mixed evaluation of a
- * context times basis evaluation + contraction on its first chunk for
- * kernels 3 to 7 with N-split counts 8 (6, 7 only), 16 and 64, each twice,
- * and keeps the fastest.  Returns the choice (variant -1 before the first
- * mixed call), its split count (0: chosen by workgroup count) and the best
- * time in ms of each variant (ms[7], 0 for variants not timed). */
+ * double buffered (6) or pipelined (7).  Kernels 3 to 5 (and the FP64
+ * kernel) visit only the 32-function blocks that screening leaves non-zero
+ * over their row tile; 1, 2, 6 and 7 compute the dense product.  The first
+ * mixed evaluation of a context times basis evaluation + contraction on its
+ * first chunk for kernels 3 to 5 with N-split counts 4, 8 and 16, each
+ * twice, and keeps the fastest.  Returns the choice (variant -1 before the
+ * first mixed call), its split count (0: chosen by workgroup count) and the
+ * best time in ms of each variant (ms[7], 0 for variants not timed). */
 int gimic_ctx_tuning(const gimic_ctx *ctx, int *variant, int *split, double *ms);
 /* Every candidate the tuner timed: codes[k] = variant * 100 + split count,
  * ms[k] its best time.  *n: capacity in, number of candidates out. */
@@ -142,6 +144,17 @@ int gimic_ctx_tuning_log(const gimic_ctx *ctx, int *n, int *codes, double *ms);
  * 0 = by workgroup count) or return to automatic selection (variant 0: the
  * next mixed evaluation tunes again). */
 int gimic_ctx_set_mixed_kernel(gimic_ctx *ctx, int variant, int split);
+
+/* Screening block-list statistics (no counterpart in the reference).  While
+ * enabled (on != 0; enabling clears the totals), every eager evaluation
+ * that runs a block-list kernel copies its row tiles' block counts to the
+ * host (a synchronising copy: use it outside timed regions) and adds them
+ * up.  out[5]: row tiles, sum of listed blocks, sum of squared listed
+ * blocks, rows per tile, blocks per dense row (lda / 32).  A tile of R rows
+ * with n listed blocks costs 2 * R * 7 * (32 n)^2 multiply-adds x 2 flops
+ * per density set. */
+int gimic_ctx_set_tile_stats(gimic_ctx *ctx, int on);
+int gimic_ctx_tile_stats(const gimic_ctx *ctx, double *out);
 
 /* Graph mode for gimic_flux_planes (no counterpart in the reference): when
  * enabled, the whole plane batch (basis evaluation, contraction and tensor

@@ -1,0 +1,92 @@
+"""GPU: screening block lists and spatial point order.
+
+Rows are sorted along a Morton curve before evaluation and each contraction
+row tile visits only the 32-function blocks that screening can leave
+non-zero over it (basis.f90:118-136 zeroes a contraction beyond its
+radius).  These tests check what that must not change: every point's
+tensor whatever the caller's order (to summation-order rounding, 1e-12
+relative), exact zeros where every block is screened, full lists when
+screening is off, and the FP64 path against the oracle on a grid where the
+lists leave most blocks out.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from oracle import oracle, sph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cage():
+    from gimic_amd import synth
+    s = synth.cage60()
+    ctx = s.context(screening=1e-8)
+    yield s, ctx
+    ctx.close()
+
+
+def _grid(n, spacing, origin):
+    from gimic_amd import synth
+    return synth.cube_grid(n, spacing, 0.0) + np.asarray(origin, dtype=np.float64)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_result_does_not_depend_on_point_order(cage, precision):
+    s, ctx = cage
+    pts = _grid(24, 0.15, [3.0, -1.8, -1.8])  # across the cage wall
+    perm = np.random.default_rng(3).permutation(len(pts))
+    print("case cage60 point order", precision, flush=True)
+    J = ctx.jtensor(pts, precision=precision)
+    Jp = ctx.jtensor(pts[perm], precision=precision)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    e = rel_err(Jp[inv], J)
+    assert e.max() <= 1e-12, e.max()
+    # the same order twice gives the same bits
+    assert np.array_equal(ctx.jtensor(pts, precision=precision), J)
+
+
+def test_lists_leave_out_blocks_and_fp64_matches_oracle(cage):
+    s, ctx = cage
+    pts = _grid(16, 0.2, [4.5, -1.5, -1.5])  # 4096 points near one side of the cage
+    ctx.set_tile_stats(True)
+    print("case cage60 block lists fp64", flush=True)
+    J = ctx.jtensor(pts, precision="fp64")
+    st = ctx.tile_stats()
+    ctx.set_tile_stats(False)
+    assert st["tile_rows"] == 128 and st["tiles"] == len(pts) // 128
+    frac = st["blocks"] / (st["tiles"] * st["nblk"])
+    assert frac < 0.8, frac  # most of the far side's blocks are screened
+    sub = np.random.default_rng(5).choice(len(pts), 48, replace=False)
+    dens = sph.to_cartesian(s.dens, s.shell_l)  # the oracle is cartesian: D_cart = T^T D T
+    ref = oracle.jtensor(s.arrays(), dens, pts[sub], screening=1e-8)
+    assert rel_err(J[sub], ref).max() <= 1e-10
+
+
+def test_points_beyond_every_radius_give_zero(cage):
+    s, ctx = cage
+    pts = _grid(8, 0.5, [60.0, 60.0, 60.0])  # 512 points far outside every screening radius
+    ctx.set_tile_stats(True)
+    for precision in ("fp64", "mixed"):
+        print("case far points", precision, flush=True)
+        J = ctx.jtensor(pts, precision=precision)
+        assert np.all(J == 0.0)
+    st = ctx.tile_stats()
+    ctx.set_tile_stats(False)
+    assert st["tiles"] > 0 and st["blocks"] == 0
+
+
+def test_unscreened_lists_are_full():
+    from gimic_amd import synth
+    s = synth.ring12()
+    pts = _grid(8, 0.6, [-2.1, -2.1, -2.1])
+    with s.context(screening=None) as ctx:
+        ctx.set_tile_stats(True)
+        J = ctx.jtensor(pts, precision="fp64")
+        st = ctx.tile_stats()
+    assert st["blocks"] == st["tiles"] * st["nblk"]
+    dens = sph.to_cartesian(s.dens, s.shell_l)
+    ref = oracle.jtensor(s.arrays(), dens, pts[:40], screening=None)
+    assert rel_err(J[:40], ref).max() <= 1e-10
