@@ -126,8 +126,9 @@ struct gimic_ctx {
     // 1 register-tiled, 2 LDS-tiled, 3 and 4 pre-split operands with
     // direct-to-LDS staging, double or triple buffered, 5 pre-split with 256
     // rows per workgroup, 6 and 7 pre-split on 32x32 MFMA blocks, double
-    // buffered or pipelined) and the timings (basis evaluation +
-    // contraction) that chose it.  Variants 0 (FP64) and 3-5 walk the
+    // buffered or pipelined, 8 pre-split with 8 wavefronts and 256 rows,
+    // double buffered) and the timings (basis evaluation + contraction)
+    // that chose it.  Variants 0 (FP64), 3-5 and 8 walk the
     // screening block lists; 1, 2, 6 and 7 compute the dense product and are
     // not among the tuner's candidates (set_mixed_kernel selects them)
     int mixed_variant = -1;
@@ -135,7 +136,7 @@ struct gimic_ctx {
     // block-list statistics (gimic_ctx_set_tile_stats): tiles, sum n, sum n^2, rows per tile
     bool tile_stats = false;
     double tile_acc[4] = {0.0, 0.0, 0.0, 0.0};
-    float tune_ms[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float tune_ms[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     std::vector<std::pair<int, float>> tune_log;  // (variant * 100 + split, ms)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
     double ms[3] = {0.0, 0.0, 0.0};
@@ -580,7 +581,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         };
         auto plan_variant = [&](int variant, int split = -1) {
             if (split < 0) split = c->mixed_split;
-            if (variant == 5) plan(256, contract_mx_cols(), split);
+            if (variant == 5 || variant >= 8) plan(256, contract_mx_cols(), split);
             else if (variant >= 3) plan(contract_mx_rows(), contract_mx_cols(), split);
             else if (variant == 2) plan(contract_mx_rows(), contract_mx_cols());
             else plan(kWgRows, kNT);
@@ -588,14 +589,16 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         auto run = [&](int variant, int split = -1) {
             plan_variant(variant, split);
             // kernels that walk block lists: FP64 and pre-split 16x16 (128 or 256 rows)
-            if (variant == 0 || (variant >= 3 && variant <= 5))
-                lists(variant == 0 ? kWgRows : variant == 5 ? 256 : contract_mx_rows());
+            if (variant == 0 || (variant >= 3 && variant <= 5) || variant >= 8)
+                lists(variant == 0 ? kWgRows
+                      : variant == 5 || variant >= 8 ? 256 : contract_mx_rows());
             if (variant == 0) hip_check(launch_contract_f64(ca, st), "contract_f64");
             else if (variant == 1) hip_check(launch_contract_bf16x6(ca, st), "contract_bf16x6");
             else if (variant == 2) hip_check(launch_contract_mx(ca, st), "contract_mx");
-            else if (variant == 3) hip_check(launch_contract_mx2(ca, 2, 2, st), "contract_mx2");
-            else if (variant == 4) hip_check(launch_contract_mx2(ca, 3, 2, st), "contract_mx2");
-            else if (variant == 5) hip_check(launch_contract_mx2(ca, 2, 4, st), "contract_mx2");
+            else if (variant == 3) hip_check(launch_contract_mx2(ca, 2, 2, 4, st), "contract_mx2");
+            else if (variant == 4) hip_check(launch_contract_mx2(ca, 3, 2, 4, st), "contract_mx2");
+            else if (variant == 5) hip_check(launch_contract_mx2(ca, 2, 4, 4, st), "contract_mx2");
+            else if (variant == 8) hip_check(launch_contract_mx2(ca, 2, 2, 8, st), "contract_mx2");
             else hip_check(launch_contract_mx3(ca, variant == 6 ? 2 : 3, st), "contract_mx3");
         };
         // basis evaluation output each contraction kernel reads
@@ -626,8 +629,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             // the fastest
             float best = 0.f;
             int best_split = 0;
-            const int cand[][2] = {{3, 4}, {3, 8}, {3, 16}, {4, 4}, {4, 8}, {4, 16},
-                                   {5, 4}, {5, 8}, {5, 16}};
+            const int cand[][2] = {{3, 4}, {4, 4}, {5, 4}, {8, 2}, {8, 4}, {8, 8}};
             c->tune_log.clear();
             for (auto &cv : cand) {
                 const int v = cv[0], sp = cv[1];
@@ -829,7 +831,7 @@ int gimic_ctx_tuning(const gimic_ctx *c, int *variant, int *split, double *ms) {
         if (!c || !variant || !split || !ms) throw Error(GIMIC_EINVAL, "null argument");
         *variant = c->mixed_variant;
         *split = c->mixed_split;
-        for (int k = 0; k < 7; k++) ms[k] = c->tune_ms[k];
+        for (int k = 0; k < 8; k++) ms[k] = c->tune_ms[k];
     });
 }
 
@@ -1010,7 +1012,7 @@ int gimic_ctx_tuning_log(const gimic_ctx *c, int *n, int *codes, double *ms) {
 int gimic_ctx_set_mixed_kernel(gimic_ctx *c, int variant, int split) {
     return guarded([&] {
         if (!c) throw Error(GIMIC_EINVAL, "null context");
-        if (variant < 0 || variant > 7) throw Error(GIMIC_EINVAL, "mixed kernel must be 0..7");
+        if (variant < 0 || variant > 8) throw Error(GIMIC_EINVAL, "mixed kernel must be 0..8");
         if (split < 0) throw Error(GIMIC_EINVAL, "split count must be >= 0");
         c->mixed_variant = variant == 0 ? -1 : variant;
         c->mixed_split = variant >= 3 ? split : 0;

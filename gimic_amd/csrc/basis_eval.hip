@@ -73,6 +73,15 @@ __host__ __device__ constexpr Comp tm_table(int L) {
     return t;
 }
 
+// Row m - L of T_l (host_basis.cpp sph_transform) combines only monomials
+// x^a y^b z^c with b odd iff m < 0, a + b of the parity of |m| and c of the
+// parity of l - |m| (its terms are x^(2t+|m|-2u-v) y^(2u+v) z^(l-2t-|m|) with
+// v odd iff m < 0); every other entry is zero and is skipped at compile time.
+__host__ __device__ constexpr bool sph_term(int L, int mi, int a, int b, int c) {
+    const int m = mi - L, am = m < 0 ? -m : m, yp = m < 0 ? 1 : 0;
+    return (a & 1) == ((am + yp) & 1) && (b & 1) == yp && (c & 1) == ((L - am) & 1);
+}
+
 __host__ __device__ constexpr int sph_off(int L) {
     return L == 2 ? 0 : L == 3 ? 30 : L == 4 ? 100 : L == 5 ? 235 : 0;
 }
@@ -131,6 +140,7 @@ __device__ void eval_shell(const EvalArgs &a, size_t row, int fout, const double
     constexpr int NC = (L + 1) * (L + 2) / 2;
     constexpr int NS = 2 * L + 1;
     constexpr Comp T = TM ? tm_table(L) : lex_table(L);
+    constexpr Comp LX = lex_table(L);  // column order of c_sph
     double px[L + 1], py[L + 1], pz[L + 1];
     px[0] = py[0] = pz[0] = 1.0;
 #pragma unroll
@@ -159,6 +169,7 @@ __device__ void eval_shell(const EvalArgs &a, size_t row, int fout, const double
         if (TRANSFORM) {
 #pragma unroll
             for (int m = 0; m < NS; m++) {
+                if (!sph_term(L, m, LX.e[c][0], LX.e[c][1], LX.e[c][2])) continue;
                 const double t = c_sph[sph_off(L) + m * NC + c];
                 acc[m][0] += t * v;
                 acc[m][1] += t * dx;

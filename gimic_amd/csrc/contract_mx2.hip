@@ -18,11 +18,15 @@
 // K-step ks; NBUF = 3 streams block ks+2 (154 KiB of LDS), so a block has
 // two K-steps to arrive from the Infinity Cache.
 //
-// Tile: 64 * KRB point rows per workgroup (4 wavefronts x 16 KRB rows), 224
-// columns per N-tile (7 matrices x 32 functions), KRB x 14 MFMA blocks per
-// wavefront.  KRB = 4 reads every B fragment from LDS once for 4 row blocks
-// (half the LDS traffic per MFMA of KRB = 2) and doubles the work per K-step
-// that hides the next step's loads; it needs 224 accumulator registers.
+// Tile: 16 * KRB * WAVES point rows per workgroup (WAVES wavefronts x 16 KRB
+// rows), 224 columns per N-tile (7 matrices x 32 functions), KRB x 14 MFMA
+// blocks per wavefront.  KRB = 4 reads every B fragment from LDS once for 4
+// row blocks (half the LDS traffic per MFMA of KRB = 2) and doubles the work
+// per K-step that hides the next step's loads; it needs 224 accumulator
+// registers.  WAVES = 8 (256 rows, two wavefronts per SIMD) lets one
+// wavefront's MFMA chains run while the other waits on a dependency or a
+// load, and each staged block feeds twice the rows.
+
 //
 // Screening.  A workgroup visits only the 32-function blocks listed for its
 // row tile (tiles.hip): the K-loop walks the list, and the listed N-blocks
@@ -49,28 +53,35 @@ constexpr int kCB = kCols / 16;             // 14 column blocks
 constexpr int kBlk = kMx2BlockElems;        // BF16 per block / LDS buffer
 constexpr int kPlane = 4 * kCols * 8;       // BF16 per part plane of a block
 constexpr int kPieces = kBlk * 2 / 1024;    // 1 KiB pieces per block (44)
-constexpr int kPiecesPerWave = kPieces / 4; // 11
 constexpr size_t kBufBytes = (size_t)kBlk * 2;
-constexpr size_t lds_q(int rows) { return (size_t)rows * kNQ * 8; }
-static_assert(kPieces % 4 == 0, "block must split evenly over 4 wavefronts");
+// pieces a wavefront copies of each block: wavefront w copies w, w + WAVES, ...
+template <int WAVES>
+constexpr int pieces_max() { return (kPieces + WAVES - 1) / WAVES; }
+__device__ __forceinline__ int pieces_of(int wave, int waves) {
+    return (kPieces - wave + waves - 1) / waves;
+}
 static_assert(3 * kPlane <= kBlk, "block too small");
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// piece i of this wavefront's share of a 44 KiB block: wavefront w copies
-// pieces w, w+4, ... (64 lanes x 16 bytes per instruction, LDS base in M0)
+// piece i of this wavefront's share of a 44 KiB block (64 lanes x 16 bytes
+// per instruction, LDS base in M0)
+template <int WAVES>
 __device__ __forceinline__ void stage_piece(const __bf16 *blk, unsigned char *buf, int wave,
                                             int lane, int i) {
-    const int piece = wave + 4 * i;
+    const int piece = wave + WAVES * i;
     const unsigned char *src = (const unsigned char *)blk + piece * 1024 + lane * 16;
     __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(buf + piece * 1024), 16, 0,
                                      0);
 }
 
+template <int WAVES>
 __device__ __forceinline__ void stage_block(const __bf16 *blk, unsigned char *buf, int wave,
                                             int lane) {
+    const int np = pieces_of(wave, WAVES);
 #pragma unroll
-    for (int i = 0; i < kPiecesPerWave; i++) stage_piece(blk, buf, wave, lane, i);
+    for (int i = 0; i < pieces_max<WAVES>(); i++)
+        if (i < np) stage_piece<WAVES>(blk, buf, wave, lane, i);
 }
 
 __device__ __forceinline__ v4f mfma6(const v8bf (&a)[3], const v8bf (&b)[3], v4f acc) {
@@ -86,16 +97,18 @@ __device__ __forceinline__ v4f mfma6(const v8bf (&a)[3], const v8bf (&b)[3], v4f
 }  // namespace
 
 // s_waitcnt immediates (gfx9 encoding: vmcnt [3:0] and [15:14], expcnt
-// [6:4], lgkmcnt [11:8]): everything, or all but the last kPiecesPerWave
-// vector-memory operations (the block copy issued last in a K-step)
+// [6:4], lgkmcnt [11:8]): everything, or all but the last n vector-memory
+// operations (the block copy issued last in a K-step)
 constexpr int kWaitAll = 0;
-constexpr int kWaitAllButBlock = (kPiecesPerWave & 15) | ((kPiecesPerWave >> 4) << 14) | (7 << 4);
-static_assert(kPiecesPerWave < 64, "vmcnt field");
+constexpr int wait_all_but(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4); }
 
-template <int NBUF, int KRB>
-__global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
-    constexpr int kRows = 64 * KRB;  // point rows per workgroup
-    constexpr int kRB = KRB;         // row blocks per wavefront
+template <int NBUF, int KRB, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArgs a) {
+    constexpr int kRows = 16 * KRB * WAVES;  // point rows per workgroup
+    constexpr int kRB = KRB;                 // row blocks per wavefront
+    constexpr int kThreads = 64 * WAVES;
+    constexpr int kPW = pieces_max<WAVES>();
+    static_assert(kPW < 64, "vmcnt field");
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char *bbuf = smem;
     double *qs = (double *)(smem + NBUF * kBufBytes);
@@ -116,7 +129,8 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
     const size_t rows = (size_t)a.rows;
     const size_t gplane = (size_t)a.gplane;
     const size_t arow = 3 * (size_t)lda;  // BF16 per split row
-    for (int i = tid; i < kRows * kNQ; i += 256) qs[i] = 0.0;
+    const int np = pieces_of(wave, WAVES);  // pieces this wavefront copies per block
+    for (int i = tid; i < kRows * kNQ; i += kThreads) qs[i] = 0.0;
 
     // lane (g, c) of row block rb: row wave*32 + rb*16 + c, k = 32 ks + 8 g .. + 7
     const __bf16 *Aw = a.as3 + (row0 + wave * 16 * kRB + c) * arow + g * 24;
@@ -135,8 +149,9 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
 #pragma unroll
             for (int cb = 0; cb < kCB; cb++) acc[rb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
 
-        stage_block(Bnt + (size_t)L[0] * kBlk, bbuf, wave, lane);
-        if (NBUF == 3 && na > 1) stage_block(Bnt + (size_t)L[1] * kBlk, bbuf + kBufBytes, wave, lane);
+        stage_block<WAVES>(Bnt + (size_t)L[0] * kBlk, bbuf, wave, lane);
+        if (NBUF == 3 && na > 1)
+            stage_block<WAVES>(Bnt + (size_t)L[1] * kBlk, bbuf + kBufBytes, wave, lane);
         v8bf an[kRB][3];
 #pragma unroll
         for (int rb = 0; rb < kRB; rb++)
@@ -164,7 +179,7 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
             // into a buffer nobody reads again) so that the count of outstanding
             // loads is the same on every iteration.
             // Vector-memory instructions (the next K-step's A fragments, two
-            // per column block from the first one on, and the block copy's 11
+            // per column block from the first one on, and the block copy's
             // pieces) are spread over the column blocks, between MFMA groups:
             // issued back to back they stall the in-order instruction stream,
             // and the matrix pipe behind it, while the vector-memory pipe
@@ -173,7 +188,7 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
             // last copy" covers the A loads.
             const bool more = ks + 1 < na;
             constexpr int kALoads = 3 * kRB;
-            constexpr int kCopyFrom = NBUF == 3 ? kCB - kPiecesPerWave : 0;
+            constexpr int kCopyFrom = NBUF == 3 ? kCB - kPW : 0;
             static_assert(NBUF != 3 || (kALoads + 1) / 2 <= kCopyFrom, "A loads must precede the copy");
             const int ahead = ks + NBUF - 1;
             const bool copy = NBUF == 3 || ahead < na;
@@ -200,8 +215,8 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
                     if (more && j < kALoads)
                         an[j / 3][j % 3] = *(const v8bf *)(Aw + (j / 3) * 16 * arow +
                                                           (size_t)a_blk * 96 + (j % 3) * 8);
-                if (copy && cb >= kCopyFrom && cb - kCopyFrom < kPiecesPerWave)
-                    stage_piece(cblk, cbuf, wave, lane, cb - kCopyFrom);
+                if (copy && cb >= kCopyFrom && cb - kCopyFrom < np)
+                    stage_piece<WAVES>(cblk, cbuf, wave, lane, cb - kCopyFrom);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int rb = 0; rb < kRB; rb++) acc[rb][cb] = mfma6(as[rb], bf[sl], acc[rb][cb]);
@@ -213,8 +228,14 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
             // visible and nobody reads the buffer the next copy overwrites
             // (a bare s_barrier: __syncthreads would add a fence that waits
             // for every outstanding load, the early copy included)
-            if (NBUF == 3) __builtin_amdgcn_s_waitcnt(kWaitAllButBlock);
-            else __builtin_amdgcn_s_waitcnt(kWaitAll);
+            if (NBUF == 3) {
+                // np is uniform per wavefront: kPW, or kPW - 1 when the
+                // pieces do not split evenly over the wavefronts
+                if (np == kPW) __builtin_amdgcn_s_waitcnt(wait_all_but(kPW));
+                else __builtin_amdgcn_s_waitcnt(wait_all_but(kPW > 1 ? kPW - 1 : 0));
+            } else {
+                __builtin_amdgcn_s_waitcnt(kWaitAll);
+            }
             __builtin_amdgcn_s_barrier();
         }
         // no copy may still be writing a buffer that the next N-tile's
@@ -277,19 +298,19 @@ __global__ __launch_bounds__(256, 1) void contract_mx2_kernel(ContractArgs a) {
     }
     __syncthreads();
     double *out = a.part + ((size_t)set * a.nsplit + split) * kNQ * rows + row0;
-    for (int i = tid; i < kRows * kNQ; i += 256) {
+    for (int i = tid; i < kRows * kNQ; i += kThreads) {
         const int q = i / kRows, rl = i % kRows;
         out[(size_t)q * rows + rl] = qs[rl * kNQ + q];
     }
 }
 
-template <int NBUF, int KRB>
+template <int NBUF, int KRB, int WAVES>
 static hipError_t launch_one(const ContractArgs &a, hipStream_t s) {
     static bool attr = false;
-    constexpr int rows = 64 * KRB;
-    const size_t lds = NBUF * kBufBytes + lds_q(rows);
+    constexpr int rows = 16 * KRB * WAVES;
+    const size_t lds = NBUF * kBufBytes + (size_t)rows * kNQ * 8;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)contract_mx2_kernel<NBUF, KRB>,
+        hipError_t e = hipFuncSetAttribute((const void *)contract_mx2_kernel<NBUF, KRB, WAVES>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         attr = true;
@@ -299,14 +320,19 @@ static hipError_t launch_one(const ContractArgs &a, hipStream_t s) {
     const long long groups = (a.rows / rows + 7) / 8;
     const long long nwg = 8 * groups * a.nsplit * a.nmatset;
     if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((contract_mx2_kernel<NBUF, KRB>), dim3((unsigned)nwg), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((contract_mx2_kernel<NBUF, KRB, WAVES>), dim3((unsigned)nwg),
+                       dim3(64 * WAVES), lds, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, hipStream_t s) {
-    if (nbuf == 2 && krb == 2) return launch_one<2, 2>(a, s);
-    if (nbuf == 3 && krb == 2) return launch_one<3, 2>(a, s);
-    if (nbuf == 2 && krb == 4) return launch_one<2, 4>(a, s);  // 3 buffers + 256-row partials exceed LDS
+hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, int waves, hipStream_t s) {
+    if (waves == 4) {
+        if (nbuf == 2 && krb == 2) return launch_one<2, 2, 4>(a, s);
+        if (nbuf == 3 && krb == 2) return launch_one<3, 2, 4>(a, s);
+        if (nbuf == 2 && krb == 4) return launch_one<2, 4, 4>(a, s);
+    } else if (waves == 8 && krb == 2 && nbuf == 2) {
+        return launch_one<2, 2, 8>(a, s);  // 3 buffers + 256-row partials exceed LDS
+    }
     return hipErrorInvalidValue;
 }
 

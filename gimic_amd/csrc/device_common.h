@@ -101,9 +101,10 @@ int contract_mx_rows();  // row tile of launch_contract_mx (a multiple of kWgRow
 int contract_mx_cols();  // functions per N-tile of launch_contract_mx
 // Pre-split variant: both operands arrive as BF16 parts, the right operand in
 // blocks that are byte images of the kernel's LDS buffer (see contract_mx2.hip)
-// nbuf 2 or 3 LDS buffers, krb 2 or 4 row blocks per wavefront (128 or 256
-// rows per workgroup; nbuf 3 only with krb 2)
-hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, hipStream_t s);
+// nbuf 2 or 3 LDS buffers, krb 2 or 4 row blocks per wavefront, 4 or 8
+// wavefronts (16 * krb * waves rows per workgroup: 4 waves with krb 2, nbuf
+// 2 or 3, or krb 4, nbuf 2; 8 waves with krb 2, nbuf 2)
+hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, int waves, hipStream_t s);
 // Pre-split operands on 32x32x16 MFMA blocks, 128 rows per workgroup (contract_mx3.hip)
 hipError_t launch_contract_mx3(const ContractArgs &a, int nbuf, hipStream_t s);
 constexpr int kMx2Cols = 224;                         // 7 matrices x 32 functions

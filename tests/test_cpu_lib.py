@@ -88,3 +88,24 @@ def test_spherical_transform_matches_independent_recursion():
             assert np.allclose(T, sph.transform(l), atol=1e-14)
         else:
             assert np.allclose(T, np.eye((l + 1) * (l + 2) // 2))
+
+
+def test_spherical_transform_zeros_follow_parity():
+    """The basis kernel skips, at compile time, every T_l entry whose
+    monomial x^a y^b z^c has the wrong parity for row m: b odd iff m < 0,
+    a + b of the parity of |m|, c of the parity of l - |m|.  Those entries
+    must be exactly zero in the table the library uploads, and the rule
+    must not be vacuous (rows keep at least one term, most entries go)."""
+    for l in range(2, 6):
+        T = np.asarray(lib.sph_transform(l))
+        comps = [(a, b, l - a - b) for a in range(l, -1, -1) for b in range(l - a, -1, -1)]
+        kept = 0
+        for mi in range(2 * l + 1):
+            m = mi - l
+            yp = 1 if m < 0 else 0
+            mask = [(a & 1) == ((abs(m) + yp) & 1) and (b & 1) == yp and (c & 1) == ((l - abs(m)) & 1)
+                    for a, b, c in comps]
+            assert all(T[mi, k] == 0.0 for k in range(len(comps)) if not mask[k]), (l, m)
+            assert any(T[mi, k] != 0.0 for k in range(len(comps)) if mask[k]), (l, m)
+            kept += sum(mask)
+        assert kept < 0.4 * T.size, (l, kept, T.size)
