@@ -368,7 +368,8 @@ def main():
     roof["kernel"] = {1: "contract_bf16x6_kernel", 2: "contract_mx_kernel",
                       3: "contract_mx2_kernel<2,2,4>", 4: "contract_mx2_kernel<3,2,4>",
                       5: "contract_mx2_kernel<2,4,4>", 6: "contract_mx3_kernel<2>",
-                      7: "contract_mx3p_kernel", 8: "contract_mx2_kernel<2,2,8>"}.get(variant, "?")
+                      7: "contract_mx3p_kernel", 8: "contract_mx2_kernel<2,2,8>",
+                      9: "contract_mx2_kernel<3,2,8>"}.get(variant, "?")
     roof["split"] = split
     roof["traffic"] = measured_traffic(roof["kernel"], split, roof["points_per_launch"])
     if roof["traffic"] is not None:

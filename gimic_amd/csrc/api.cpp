@@ -126,9 +126,9 @@ struct gimic_ctx {
     // 1 register-tiled, 2 LDS-tiled, 3 and 4 pre-split operands with
     // direct-to-LDS staging, double or triple buffered, 5 pre-split with 256
     // rows per workgroup, 6 and 7 pre-split on 32x32 MFMA blocks, double
-    // buffered or pipelined, 8 pre-split with 8 wavefronts and 256 rows,
-    // double buffered) and the timings (basis evaluation + contraction)
-    // that chose it.  Variants 0 (FP64), 3-5 and 8 walk the
+    // buffered or pipelined, 8 and 9 pre-split with 8 wavefronts and 256
+    // rows, double or triple buffered) and the timings (basis evaluation +
+    // contraction) that chose it.  Variants 0 (FP64), 3-5, 8 and 9 walk the
     // screening block lists; 1, 2, 6 and 7 compute the dense product and are
     // not among the tuner's candidates (set_mixed_kernel selects them)
     int mixed_variant = -1;
@@ -136,7 +136,7 @@ struct gimic_ctx {
     // block-list statistics (gimic_ctx_set_tile_stats): tiles, sum n, sum n^2, rows per tile
     bool tile_stats = false;
     double tile_acc[4] = {0.0, 0.0, 0.0, 0.0};
-    float tune_ms[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float tune_ms[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     std::vector<std::pair<int, float>> tune_log;  // (variant * 100 + split, ms)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
     double ms[3] = {0.0, 0.0, 0.0};
@@ -586,8 +586,13 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             else if (variant == 2) plan(contract_mx_rows(), contract_mx_cols());
             else plan(kWgRows, kNT);
         };
+        // pre-split kernels add row-major partial sums into a cleared buffer
+        auto row_major = [](int variant) { return (variant >= 3 && variant <= 5) || variant >= 8; };
         auto run = [&](int variant, int split = -1) {
             plan_variant(variant, split);
+            if (row_major(variant))
+                hip_check(launch_fill_zero(ca.part, (size_t)nmatset * ca.nsplit * kNQ * rows, st),
+                          "fill_zero");
             // kernels that walk block lists: FP64 and pre-split 16x16 (128 or 256 rows)
             if (variant == 0 || (variant >= 3 && variant <= 5) || variant >= 8)
                 lists(variant == 0 ? kWgRows
@@ -599,6 +604,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             else if (variant == 4) hip_check(launch_contract_mx2(ca, 3, 2, 4, st), "contract_mx2");
             else if (variant == 5) hip_check(launch_contract_mx2(ca, 2, 4, 4, st), "contract_mx2");
             else if (variant == 8) hip_check(launch_contract_mx2(ca, 2, 2, 8, st), "contract_mx2");
+            else if (variant == 9) hip_check(launch_contract_mx2(ca, 3, 2, 8, st), "contract_mx2");
             else hip_check(launch_contract_mx3(ca, variant == 6 ? 2 : 3, st), "contract_mx3");
         };
         // basis evaluation output each contraction kernel reads
@@ -629,7 +635,8 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             // the fastest
             float best = 0.f;
             int best_split = 0;
-            const int cand[][2] = {{3, 4}, {4, 4}, {5, 4}, {8, 2}, {8, 4}, {8, 8}};
+            const int cand[][2] = {{3, 4}, {4, 4}, {5, 4}, {8, 2}, {8, 4}, {8, 8},
+                                   {9, 2}, {9, 4}, {9, 8}};
             c->tune_log.clear();
             for (auto &cv : cand) {
                 const int v = cv[0], sp = cv[1];
@@ -672,6 +679,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         FinalizeArgs fa{};
         fa.part = c->part.as<double>();
         fa.nsplit = nsplit;
+        fa.part_rows = row_major(variant) ? 1 : 0;
         fa.rows = rows;
         fa.npts = n_c;
         fa.nmatset = nmatset;
@@ -831,7 +839,7 @@ int gimic_ctx_tuning(const gimic_ctx *c, int *variant, int *split, double *ms) {
         if (!c || !variant || !split || !ms) throw Error(GIMIC_EINVAL, "null argument");
         *variant = c->mixed_variant;
         *split = c->mixed_split;
-        for (int k = 0; k < 8; k++) ms[k] = c->tune_ms[k];
+        for (int k = 0; k < 9; k++) ms[k] = c->tune_ms[k];
     });
 }
 
@@ -1012,7 +1020,7 @@ int gimic_ctx_tuning_log(const gimic_ctx *c, int *n, int *codes, double *ms) {
 int gimic_ctx_set_mixed_kernel(gimic_ctx *c, int variant, int split) {
     return guarded([&] {
         if (!c) throw Error(GIMIC_EINVAL, "null context");
-        if (variant < 0 || variant > 8) throw Error(GIMIC_EINVAL, "mixed kernel must be 0..8");
+        if (variant < 0 || variant > 9) throw Error(GIMIC_EINVAL, "mixed kernel must be 0..9");
         if (split < 0) throw Error(GIMIC_EINVAL, "split count must be >= 0");
         c->mixed_variant = variant == 0 ? -1 : variant;
         c->mixed_split = variant >= 3 ? split : 0;

@@ -119,9 +119,7 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs a) {
 #pragma unroll
                     for (int m = 0; m < 3; m++) q[13 + m + 3 * d] = G[m] * acc[rb][4 + d][r];
 #pragma unroll
-                for (int off = 1; off < 16; off <<= 1)
-#pragma unroll
-                    for (int i = 0; i < kNQ; i++) q[i] += __shfl_xor(q[i], off);
+                for (int i = 0; i < kNQ; i++) q[i] = row16_sum(q[i]);
 #pragma unroll
                 for (int i = 0; i < kNQ; i++)
                     if ((i & 15) == c) qs[rl * kNQ + i] += q[i];

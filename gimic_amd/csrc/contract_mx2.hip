@@ -26,6 +26,15 @@
 // registers.  WAVES = 8 (256 rows, two wavefronts per SIMD) lets one
 // wavefront's MFMA chains run while the other waits on a dependency or a
 // load, and each staged block feeds twice the rows.
+//
+// Per-point partial sums.  After the butterfly over the 16 columns of a lane
+// group every lane holds all kNQ sums of its row; lane c adds sums c and
+// c + 16 (FP64) to the row-major partial buffer [set][split][row][kNQ] with
+// no-return atomic adds, 16 lanes of a group covering 128 contiguous bytes.
+// Each address belongs to one lane of one workgroup, which adds its N-tiles
+// in order to a buffer cleared before the launch, so the sums are the same,
+// bit for bit, on every run, and no LDS holds them (three 44 KiB buffers fit
+// with 256-row tiles).
 
 //
 // Screening.  A workgroup visits only the 32-function blocks listed for its
@@ -106,12 +115,10 @@ template <int NBUF, int KRB, int WAVES>
 __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArgs a) {
     constexpr int kRows = 16 * KRB * WAVES;  // point rows per workgroup
     constexpr int kRB = KRB;                 // row blocks per wavefront
-    constexpr int kThreads = 64 * WAVES;
     constexpr int kPW = pieces_max<WAVES>();
     static_assert(kPW < 64, "vmcnt field");
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char *bbuf = smem;
-    double *qs = (double *)(smem + NBUF * kBufBytes);
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -130,7 +137,8 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
     const size_t gplane = (size_t)a.gplane;
     const size_t arow = 3 * (size_t)lda;  // BF16 per split row
     const int np = pieces_of(wave, WAVES);  // pieces this wavefront copies per block
-    for (int i = tid; i < kRows * kNQ; i += kThreads) qs[i] = 0.0;
+    // this workgroup's partial sums, row-major [row][kNQ]
+    double *pout = a.part + ((size_t)set * a.nsplit + split) * kNQ * rows;
 
     // lane (g, c) of row block rb: row wave*32 + rb*16 + c, k = 32 ks + 8 g .. + 7
     const __bf16 *Aw = a.as3 + (row0 + wave * 16 * kRB + c) * arow + g * 24;
@@ -286,21 +294,19 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
                         for (int m = 0; m < 3; m++) q[13 + m + 3 * d] += G[m] * acc[rb][2 * (4 + d) + h][r];
                 }
 #pragma unroll
-                for (int off = 1; off < 16; off <<= 1)
+                for (int i = 0; i < kNQ; i++) q[i] = row16_sum(q[i]);
+                float s0 = 0.f, s1 = 0.f;
 #pragma unroll
-                    for (int i = 0; i < kNQ; i++) q[i] += __shfl_xor(q[i], off);
-#pragma unroll
-                for (int i = 0; i < kNQ; i++)
-                    if ((i & 15) == c) qs[rl * kNQ + i] += (double)q[i];
+                for (int i = 0; i < kNQ; i++) {
+                    if (i < 16) s0 = i == c ? q[i] : s0;
+                    else s1 = i - 16 == c ? q[i] : s1;
+                }
+                double *o = pout + (row0 + rl) * kNQ;
+                unsafeAtomicAdd(o + c, (double)s0);
+                if (c + 16 < kNQ) unsafeAtomicAdd(o + c + 16, (double)s1);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-    }
-    __syncthreads();
-    double *out = a.part + ((size_t)set * a.nsplit + split) * kNQ * rows + row0;
-    for (int i = tid; i < kRows * kNQ; i += kThreads) {
-        const int q = i / kRows, rl = i % kRows;
-        out[(size_t)q * rows + rl] = qs[rl * kNQ + q];
     }
 }
 
@@ -308,7 +314,7 @@ template <int NBUF, int KRB, int WAVES>
 static hipError_t launch_one(const ContractArgs &a, hipStream_t s) {
     static bool attr = false;
     constexpr int rows = 16 * KRB * WAVES;
-    const size_t lds = NBUF * kBufBytes + (size_t)rows * kNQ * 8;
+    const size_t lds = NBUF * kBufBytes;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void *)contract_mx2_kernel<NBUF, KRB, WAVES>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -330,8 +336,9 @@ hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, int wav
         if (nbuf == 2 && krb == 2) return launch_one<2, 2, 4>(a, s);
         if (nbuf == 3 && krb == 2) return launch_one<3, 2, 4>(a, s);
         if (nbuf == 2 && krb == 4) return launch_one<2, 4, 4>(a, s);
-    } else if (waves == 8 && krb == 2 && nbuf == 2) {
-        return launch_one<2, 2, 8>(a, s);  // 3 buffers + 256-row partials exceed LDS
+    } else if (waves == 8 && krb == 2) {
+        if (nbuf == 2) return launch_one<2, 2, 8>(a, s);
+        if (nbuf == 3) return launch_one<3, 2, 8>(a, s);
     }
     return hipErrorInvalidValue;
 }

@@ -50,6 +50,33 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
     return __uint_as_float(((uint32_t)h) << 16);
 }
 
+// Sum over the 16 lanes of a DPP row (lanes 16j .. 16j+15), left in every
+// lane: partners 1 and 2 apart within each quad, then the half-row mirror
+// (lane i with 7 - i) joins quads 0-1 and 2-3, the row mirror (i with
+// 15 - i) the two halves.  Each step adds the same pairs as a xor
+// butterfly over distances 1, 2, 4, 8 (the lanes of a quad, of a half-row,
+// already agree), so the sums are bit for bit those of the butterfly, with
+// one VALU instruction per step and no LDS crossbar traffic.
+#define GIMIC_DPP(x, ctrl) \
+    __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), (ctrl), 0xF, 0xF, false))
+__device__ __forceinline__ float row16_sum(float x) {
+    x += GIMIC_DPP(x, 0xB1);   // quad_perm [1,0,3,2]
+    x += GIMIC_DPP(x, 0x4E);   // quad_perm [2,3,0,1]
+    x += GIMIC_DPP(x, 0x141);  // row_half_mirror
+    x += GIMIC_DPP(x, 0x140);  // row_mirror
+    return x;
+}
+#define GIMIC_DPP_D(x, ctrl)                                                                    \
+    __hiloint2double(__builtin_amdgcn_update_dpp(0, __double2hiint(x), (ctrl), 0xF, 0xF, false), \
+                     __builtin_amdgcn_update_dpp(0, __double2loint(x), (ctrl), 0xF, 0xF, false))
+__device__ __forceinline__ double row16_sum(double x) {
+    x += GIMIC_DPP_D(x, 0xB1);
+    x += GIMIC_DPP_D(x, 0x4E);
+    x += GIMIC_DPP_D(x, 0x141);
+    x += GIMIC_DPP_D(x, 0x140);
+    return x;
+}
+
 // launchers (defined in the .hip files)
 struct EvalArgs {
     const double *pts;   // 3*npts of this chunk
@@ -87,7 +114,7 @@ struct ContractArgs {
     const __bf16 *bs3;   // split right operand, LDS-image blocks per (set, N-tile, K-step)
     const double *rf64;  // [3][ncol_pad] atom coordinate of each function
     const float *rf32;
-    double *part;        // [nmatset][nsplit][kNQ][rows]
+    double *part;        // [nmatset][nsplit][kNQ][rows]; pre-split kernels: [nmatset][nsplit][rows][kNQ]
     // screening block lists per row tile (kernels that walk them): bcount[t]
     // 32-function blocks, listed ascending in blist[t * nblk ...]; split s of
     // a tile takes the listed N-blocks [s * count / nsplit, (s+1) * count / nsplit)
@@ -103,7 +130,8 @@ int contract_mx_cols();  // functions per N-tile of launch_contract_mx
 // blocks that are byte images of the kernel's LDS buffer (see contract_mx2.hip)
 // nbuf 2 or 3 LDS buffers, krb 2 or 4 row blocks per wavefront, 4 or 8
 // wavefronts (16 * krb * waves rows per workgroup: 4 waves with krb 2, nbuf
-// 2 or 3, or krb 4, nbuf 2; 8 waves with krb 2, nbuf 2)
+// 2 or 3, or krb 4, nbuf 2; 8 waves with krb 2, nbuf 2 or 3).  Adds its
+// partial sums to a.part, row-major, which must be zero before the launch.
 hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, int waves, hipStream_t s);
 // Pre-split operands on 32x32x16 MFMA blocks, 128 rows per workgroup (contract_mx3.hip)
 hipError_t launch_contract_mx3(const ContractArgs &a, int nbuf, hipStream_t s);
@@ -111,7 +139,8 @@ constexpr int kMx2Cols = 224;                         // 7 matrices x 32 functio
 constexpr int kMx2BlockElems = 22528;                 // 3*4*224*8 = 21504, padded to 44 KiB
 
 struct FinalizeArgs {
-    const double *part;  // [nmatset][nsplit][kNQ][rows]
+    const double *part;  // [nmatset][nsplit][kNQ][rows], or [nmatset][nsplit][rows][kNQ] if part_rows
+    int part_rows;
     int nsplit, rows, npts, nmatset;
     const double *pts;   // chunk points in row order
     const int *perm;     // caller index (within the chunk) of each row, or null
@@ -127,6 +156,8 @@ struct FinalizeArgs {
     double *contrib_acid; // ACID contribution w_p dT^2 or null
 };
 hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s);
+// p[0..n) = 0 (a kernel, so it may be captured into a graph)
+hipError_t launch_fill_zero(double *p, size_t n, hipStream_t s);
 
 // deterministic per-plane sums of contributions: out[3*plane] = net, pos, neg
 hipError_t launch_plane_reduce(const double *contrib, const long long *offs, int nplanes,

@@ -32,9 +32,10 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
 #pragma unroll
         for (int i = 0; i < kNQ; i++) q[i] = 0.0;
         for (int s = 0; s < a.nsplit; s++) {
-            const double *src = a.part + ((size_t)set * a.nsplit + s) * kNQ * (size_t)a.rows + p;
+            const double *src = a.part + ((size_t)set * a.nsplit + s) * kNQ * (size_t)a.rows;
 #pragma unroll
-            for (int i = 0; i < kNQ; i++) q[i] += src[(size_t)i * a.rows];
+            for (int i = 0; i < kNQ; i++)
+                q[i] += a.part_rows ? src[(size_t)p * kNQ + i] : src[(size_t)i * a.rows + p];
         }
         // q[1+c] = E_c = rho r_c - C_c, C_c = sum_nu Y0 phi R_nu,c.  With all
         // three terms on, eps_bmd C_d + eps_mbc rho r_c = eps_mbd E_d, so the
@@ -149,6 +150,18 @@ __global__ __launch_bounds__(256) void plane_reduce_kernel(const double *contrib
         out[3 * pl + 1] = s[1][0];
         out[3 * pl + 2] = s[2][0];
     }
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(double *p, long long n) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        p[i] = 0.0;
+}
+
+hipError_t launch_fill_zero(double *p, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, (long long)n);
+    return hipGetLastError();
 }
 
 hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s) {

@@ -219,9 +219,9 @@ class Context:
 
     def tuning(self):
         """(chosen mixed kernel variant, its N-split count, [best ms of
-        variants 1 to 8, 0 where not timed])"""
+        variants 1 to 9, 0 where not timed])"""
         v, sp = ctypes.c_int(), ctypes.c_int()
-        ms = np.zeros(8)
+        ms = np.zeros(9)
         check(self._lib.gimic_ctx_tuning(self._h, ctypes.byref(v), ctypes.byref(sp), ptr(ms)))
         return v.value, sp.value, ms.tolist()
 
@@ -237,8 +237,8 @@ class Context:
         """fix the mixed contraction kernel (1 register-tiled, 2 LDS-tiled,
         3/4 pre-split double/triple buffered, 5 pre-split with 256-row
         tiles, 6/7 pre-split on 32x32 MFMA blocks double buffered/pipelined,
-        8 pre-split with 8 wavefronts and 256-row tiles double buffered;
-        3-5 and 8 visit only the blocks screening leaves non-zero,
+        8/9 pre-split with 8 wavefronts and 256-row tiles double/triple
+        buffered; 3-5, 8 and 9 visit only the blocks screening leaves non-zero,
         with `split` N-ranges, 0 = by workgroup count) or 0 for automatic
         selection"""
         check(self._lib.gimic_ctx_set_mixed_kernel(self._h, int(variant), int(split)))

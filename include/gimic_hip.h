@@ -129,21 +129,21 @@ int gimic_ctx_get_timing(gimic_ctx *ctx, double *ms, long long *launches);
  * MFMA blocks double (3) or triple (4) buffered with 128 rows per workgroup
  * or double buffered with 256 rows (5), pre-split on 32x32 MFMA blocks
  * double buffered (6) or pipelined (7), pre-split on 16x16 MFMA blocks with
- * 8 wavefronts and 256 rows per workgroup, double buffered (8).  Kernels 3
- * to 5 and 8 (and the FP64 kernel) visit only the
+ * 8 wavefronts and 256 rows per workgroup, double (8) or triple (9)
+ * buffered.  Kernels 3 to 5, 8 and 9 (and the FP64 kernel) visit only the
  * 32-function blocks that screening leaves non-zero over their row tile;
  * 1, 2, 6 and 7 compute the dense product.  The first mixed evaluation of a
  * context times basis evaluation + contraction on its first chunk for
- * kernels 3, 4 and 5 (N-split count 4) and 8 (N-split counts 2, 4 and 8),
- * each twice, and keeps the fastest.  Returns the choice (variant
+ * kernels 3, 4 and 5 (N-split count 4) and 8 and 9 (N-split counts 2, 4
+ * and 8), each twice, and keeps the fastest.  Returns the choice (variant
  * -1 before the first mixed call), its split count (0: chosen by workgroup
- * count) and the best time in ms of each variant (ms[8], 0 for variants not
+ * count) and the best time in ms of each variant (ms[9], 0 for variants not
  * timed). */
 int gimic_ctx_tuning(const gimic_ctx *ctx, int *variant, int *split, double *ms);
 /* Every candidate the tuner timed: codes[k] = variant * 100 + split count,
  * ms[k] its best time.  *n: capacity in, number of candidates out. */
 int gimic_ctx_tuning_log(const gimic_ctx *ctx, int *n, int *codes, double *ms);
-/* Fix the mixed-precision kernel (1 to 8; for 3 to 8 also the N-split count,
+/* Fix the mixed-precision kernel (1 to 9; for 3 to 9 also the N-split count,
  * 0 = by workgroup count) or return to automatic selection (variant 0: the
  * next mixed evaluation tunes again). */
 int gimic_ctx_set_mixed_kernel(gimic_ctx *ctx, int variant, int split);

@@ -154,7 +154,7 @@ def test_mixed_precision_within_stated_tolerance(ring12, c4h4):
 @pytest.mark.parametrize("variant,split", [(1, 0), (2, 0), (3, 0), (3, 1), (3, 2), (3, 3),
                                            (4, 0), (4, 1), (4, 3), (5, 0), (5, 2),
                                            (6, 0), (6, 1), (6, 3), (7, 0), (7, 1),
-                                           (8, 0), (8, 1), (8, 2), (8, 3)])
+                                           (8, 0), (8, 1), (8, 2), (9, 0), (9, 1), (9, 3)])
 def test_each_mixed_kernel_within_tolerance(c4h4, variant, split):
     """every mixed contraction kernel (register-tiled, LDS-tiled, pre-split
     with several N-split counts; 1000 points = 8 row tiles, 1300 = 11, so
