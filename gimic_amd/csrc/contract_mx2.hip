@@ -119,6 +119,7 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
     static_assert(kPW < 64, "vmcnt field");
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned char *bbuf = smem;
+    float *rps = (float *)(smem + NBUF * kBufBytes);  // [kRows][3] point coordinates (FP32)
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -139,6 +140,14 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
     const int np = pieces_of(wave, WAVES);  // pieces this wavefront copies per block
     // this workgroup's partial sums, row-major [row][kNQ]
     double *pout = a.part + ((size_t)set * a.nsplit + split) * kNQ * rows;
+    // the epilogue's point coordinates, once per workgroup (visible after
+    // the first N-tile's barrier, before any epilogue)
+    for (int i = tid; i < kRows; i += 64 * WAVES) {
+        const size_t p = row0 + i;
+        const bool valid = p < (size_t)a.npts;
+#pragma unroll
+        for (int d = 0; d < 3; d++) rps[3 * i + d] = valid ? (float)a.pts[3 * p + d] : 0.f;
+    }
 
     // lane (g, c) of row block rb: row wave*32 + rb*16 + c, k = 32 ks + 8 g .. + 7
     const __bf16 *Aw = a.as3 + (row0 + wave * 16 * kRB + c) * arow + g * 24;
@@ -254,32 +263,46 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
         }
 
         // epilogue: lane (g, c) holds Y[row 4g + r][col c] of every block;
-        // column block cb = 2 * matrix + half covers functions nt*32 + 16*half + c
+        // column block cb = 2 * matrix + half covers functions nt*32 + 16*half + c.
+        // The global loads (phi parts and gradients) of each row are issued
+        // one row ahead, so their latency overlaps the previous row's work.
         float R[2][3];
 #pragma unroll
         for (int h = 0; h < 2; h++)
 #pragma unroll
             for (int d = 0; d < 3; d++) R[h][d] = a.rf32[(size_t)d * a.ncol_pad + nt * kNu + 16 * h + c];
+        constexpr int kIt = 4 * kRB;  // rows per lane: it = 4 rb + r
+        auto row_of = [&](int it) { return wave * 16 * kRB + (it >> 2) * 16 + 4 * g + (it & 3); };
+        __bf16 pc[2][3], pn[2][3];
+        float Gc[2][3], Gn[2][3];
+        auto load_row = [&](int it, __bf16 (&pv)[2][3], float (&Gv)[2][3]) {
+            const size_t p = row0 + row_of(it);
 #pragma unroll
-        for (int rb = 0; rb < kRB; rb++) {
+            for (int h = 0; h < 2; h++) {
+                const int col = nt * kNu + 16 * h + c;
+                const __bf16 *pe = a.as3 + p * arow + (col >> 3) * 24 + (col & 7);
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int rl = wave * 16 * kRB + rb * 16 + 4 * g + r;
-                const size_t p = row0 + rl;
-                const bool valid = p < (size_t)a.npts;
-                const float rp[3] = {valid ? (float)a.pts[3 * p] : 0.f,
-                                     valid ? (float)a.pts[3 * p + 1] : 0.f,
-                                     valid ? (float)a.pts[3 * p + 2] : 0.f};
+                for (int k = 0; k < 3; k++) pv[h][k] = pe[8 * k];
+                const size_t e = p * lda + col;
+#pragma unroll
+                for (int d = 0; d < 3; d++) Gv[h][d] = a.g32[d * gplane + e];
+            }
+        };
+        load_row(0, pc, Gc);
+#pragma unroll
+        for (int it = 0; it < kIt; it++) {
+            const int rb = it >> 2, r = it & 3;
+            {
+                if (it + 1 < kIt) load_row(it + 1, pn, Gn);
+                const int rl = row_of(it);
+                const float rp[3] = {rps[3 * rl], rps[3 * rl + 1], rps[3 * rl + 2]};
                 float q[kNQ];
 #pragma unroll
                 for (int i = 0; i < kNQ; i++) q[i] = 0.f;
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                    const int col = nt * kNu + 16 * h + c;
-                    const __bf16 *pe = a.as3 + p * arow + (col >> 3) * 24 + (col & 7);
-                    const float ph = ((float)pe[0] + (float)pe[8]) + (float)pe[16];
-                    const size_t e = p * lda + col;
-                    const float G[3] = {a.g32[e], a.g32[gplane + e], a.g32[2 * gplane + e]};
+                    const float ph = ((float)pc[h][0] + (float)pc[h][1]) + (float)pc[h][2];
+                    const float *G = Gc[h];
                     const float t0 = acc[rb][h][r] * ph;
                     q[0] += t0;
 #pragma unroll
@@ -304,8 +327,15 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
                 double *o = pout + (row0 + rl) * kNQ;
                 unsafeAtomicAdd(o + c, (double)s0);
                 if (c + 16 < kNQ) unsafeAtomicAdd(o + c + 16, (double)s1);
-                __builtin_amdgcn_sched_barrier(0);
             }
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    pc[h][k] = pn[h][k];
+                    Gc[h][k] = Gn[h][k];
+                }
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
@@ -314,7 +344,7 @@ template <int NBUF, int KRB, int WAVES>
 static hipError_t launch_one(const ContractArgs &a, hipStream_t s) {
     static bool attr = false;
     constexpr int rows = 16 * KRB * WAVES;
-    const size_t lds = NBUF * kBufBytes;
+    const size_t lds = NBUF * kBufBytes + (size_t)rows * 3 * sizeof(float);
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void *)contract_mx2_kernel<NBUF, KRB, WAVES>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
