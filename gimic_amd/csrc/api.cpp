@@ -223,21 +223,32 @@ void setup_device(gimic_ctx *c) {
             pos += nout_of(s);
         }
     }
+    // Device shell array: ordered by l, then by output position.  The basis
+    // kernels hand consecutive shells of a point to consecutive lanes, so a
+    // wavefront then runs one l branch (one polynomial and transform) at a
+    // time instead of all of them with most lanes masked off.
+    std::vector<int> dorder(nsh), dpos(nsh);
+    for (int i = 0; i < nsh; i++) dorder[i] = i;
+    std::stable_sort(dorder.begin(), dorder.end(), [&](int x, int y) {
+        if (b.shells[x].l != b.shells[y].l) return b.shells[x].l < b.shells[y].l;
+        return fnew[x] < fnew[y];
+    });
+    for (int k = 0; k < nsh; k++) dpos[dorder[k]] = k;
     const int nblk = c->lda / 32;
-    std::vector<int> blk_beg(nblk + 1, 0), blk_sh;
+    std::vector<int> blk_beg(nblk + 1, 0), blk_sh;  // entries index the device array
     for (int k = 0; k < nblk; k++) {
         blk_beg[k] = (int)blk_sh.size();
         for (int si : order) {
             const int f0 = fnew[si], f1 = f0 + nout_of(b.shells[si]);
-            if (f0 < 32 * (k + 1) && f1 > 32 * k) blk_sh.push_back(si);
+            if (f0 < 32 * (k + 1) && f1 > 32 * k) blk_sh.push_back(dpos[si]);
         }
     }
     blk_beg[nblk] = (int)blk_sh.size();
     if (blk_sh.empty()) blk_sh.push_back(0);
     std::vector<DevShell> sh;
-    for (int i = 0; i < nsh; i++) {
-        const Shell &s = b.shells[i];
-        sh.push_back(DevShell{s.atom, s.l, s.nprim, s.poff, fnew[i], nout_of(s), s.radius});
+    for (int k = 0; k < nsh; k++) {
+        const Shell &s = b.shells[dorder[k]];
+        sh.push_back(DevShell{s.atom, s.l, s.nprim, s.poff, fnew[dorder[k]], nout_of(s), s.radius});
     }
     hipStream_t st = c->stream;
     auto up = [&](DevBuf &d, const void *src, size_t n) {
