@@ -43,13 +43,17 @@
 // buffers rotate over list positions, so the pipeline is the same as for a
 // dense walk.
 //
-// Workgroup order.  Every workgroup re-reads its 128 rows of A once per
-// N-tile, so A traffic is what limits this kernel.  The grid is 1-D and
-// decoded so that the nsplit workgroups that share a row tile (one per
-// N-range) are dispatched back to back on the same XCD: the hardware hands
-// workgroup i to XCD i mod 8, so slot s = i / 8 of XCD x runs row tile
-// 8 * (s / nsplit) + x with split s % nsplit.  They stream the same A rows
-// at the same time and all but one read them from that XCD's L2.
+// Workgroup order.  Every workgroup re-reads its rows of A once per N-tile,
+// and every row tile streams the B blocks of its listed (N, K) pairs.  The
+// grid is 1-D and decoded so that (1) the nsplit workgroups that share a row
+// tile (one per N-range) are dispatched back to back on the same XCD: the
+// hardware hands workgroup i to XCD i mod 8, so slot s = i / 8 of XCD x runs
+// the (s / nsplit)-th row tile of that XCD with split s % nsplit, and they
+// read the same A rows at the same time, all but one from the XCD's L2; and
+// (2) an XCD takes runs of kTileRun consecutive row tiles (runs go to the
+// XCDs in turn), so the tiles it runs together are neighbours along the
+// points' Morton order with nearly the same block lists, and their equal
+// splits stream the same B blocks through its L2 as well.
 #include "device_common.h"
 
 namespace gimic {
@@ -105,6 +109,13 @@ __device__ __forceinline__ v4f mfma6(const v8bf (&a)[3], const v8bf (&b)[3], v4f
 
 }  // namespace
 
+// Row tiles per XCD: runs of kTileRun consecutive tiles go to the XCDs in
+// turn, enough runs to cover every tile
+constexpr int kTileRun = 8;
+__host__ __device__ constexpr int tile_groups(int ntiles) {
+    return (ntiles + 8 * kTileRun - 1) / (8 * kTileRun) * kTileRun;
+}
+
 // s_waitcnt immediates (gfx9 encoding: vmcnt [3:0] and [15:14], expcnt
 // [6:4], lgkmcnt [11:8]): everything, or all but the last n vector-memory
 // operations (the block copy issued last in a K-step)
@@ -125,12 +136,13 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = lane >> 4, c = lane & 15;
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int groups = (a.rows / kRows + 7) >> 3;  // groups of 8 row tiles
+    const int groups = tile_groups(a.rows / kRows);  // row tiles per XCD
     const int per_set = groups * a.nsplit;
     const int set = slot / per_set;
     const int split = slot % a.nsplit;
-    const int rtile = ((slot % per_set) / a.nsplit) * 8 + xcd;
-    if (rtile >= a.rows / kRows) return;  // padding of the last group
+    const int jt = (slot % per_set) / a.nsplit;
+    const int rtile = (jt / kTileRun) * 8 * kTileRun + xcd * kTileRun + jt % kTileRun;
+    if (rtile >= a.rows / kRows) return;  // padding of the last run
     const size_t row0 = (size_t)rtile * kRows;
     const int lda = a.lda;
     const int nks = lda / 32;
@@ -353,7 +365,7 @@ static hipError_t launch_one(const ContractArgs &a, hipStream_t s) {
     }
     if (a.rows % rows || a.lda % 32 || a.ncol_pad != a.lda || a.ntiles * kNu != a.ncol_pad)
         return hipErrorInvalidValue;
-    const long long groups = (a.rows / rows + 7) / 8;
+    const long long groups = tile_groups(a.rows / rows);
     const long long nwg = 8 * groups * a.nsplit * a.nmatset;
     if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL((contract_mx2_kernel<NBUF, KRB, WAVES>), dim3((unsigned)nwg),

@@ -33,9 +33,19 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
         for (int i = 0; i < kNQ; i++) q[i] = 0.0;
         for (int s = 0; s < a.nsplit; s++) {
             const double *src = a.part + ((size_t)set * a.nsplit + s) * kNQ * (size_t)a.rows;
+            if (a.part_rows) {
+                // a row's kNQ sums are 176 contiguous, 16-byte aligned bytes
+                const double2 *v = (const double2 *)(src + (size_t)p * kNQ);
 #pragma unroll
-            for (int i = 0; i < kNQ; i++)
-                q[i] += a.part_rows ? src[(size_t)p * kNQ + i] : src[(size_t)i * a.rows + p];
+                for (int j = 0; j < kNQ / 2; j++) {
+                    const double2 t = v[j];
+                    q[2 * j] += t.x;
+                    q[2 * j + 1] += t.y;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < kNQ; i++) q[i] += src[(size_t)i * a.rows + p];
+            }
         }
         // q[1+c] = E_c = rho r_c - C_c, C_c = sum_nu Y0 phi R_nu,c.  With all
         // three terms on, eps_bmd C_d + eps_mbc rho r_c = eps_mbd E_d, so the
