@@ -18,6 +18,8 @@ Other workloads (one JSON line each; not the headline):
                          charge and spin current on a 192x192x96 grid
     --workload surrogate learned J field of the 60-atom cage on a 256^3 grid
                          with exact-kernel fallback
+    --workload property  shielding of all 60 nuclei and the magnetizability of
+                         the cage on per-atom product grids (jfield.f90:584-929)
 """
 import argparse
 import json
@@ -261,6 +263,64 @@ def bench_openshell(args):
     }
 
 
+def atom_product_grids(coords, rmax=6.0, nr=24, nt=12, nphi=24):
+    """per atom a product grid: Gauss-Legendre in r on [0, rmax] and in
+    cos(theta), trapezoid in phi (a stand-in for numgrid's per-atom blocks)"""
+    xr, wr = np.polynomial.legendre.leggauss(nr)
+    rr = 0.5 * rmax * (xr + 1.0)
+    wr = 0.5 * rmax * wr * rr ** 2
+    ct, wt = np.polynomial.legendre.leggauss(nt)
+    ph = 2.0 * np.pi * np.arange(nphi) / nphi
+    R, CT, PH = np.meshgrid(rr, ct, ph, indexing="ij")
+    W = (wr[:, None, None] * wt[None, :, None] * (2.0 * np.pi / nphi)).repeat(nphi, axis=2)
+    st = np.sqrt(1.0 - CT ** 2)
+    local = np.stack([R * st * np.cos(PH), R * st * np.sin(PH), R * CT], axis=-1).reshape(-1, 3)
+    pts = np.concatenate([local + c for c in coords])
+    return pts, np.tile(W.reshape(-1), len(coords)), np.full(len(coords), local.shape[0])
+
+
+def bench_property(args):
+    """shielding of all 60 nuclei + magnetizability (jfield.f90:584-929) on
+    per-atom product grids of the cage: J at every point and the 61 weighted
+    sums per point block; host arrays in and out, as the drop-in reads them
+    from numgrid files"""
+    from gimic_amd import synth
+    system = synth.cage60()
+    ctx = system.context(screening=1e-8)
+    coords = np.asarray(system.coords)
+    pts, w, sizes = atom_product_grids(coords)
+    npts = len(pts)
+    ms, res = {}, {}
+    for prec in ("mixed", "fp64"):
+        for _ in range(args.warmup):
+            ctx.properties(pts, w, sizes, coords, precision=prec)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res[prec] = ctx.properties(pts, w, sizes, coords, precision=prec)
+        ms[prec] = 1e3 * (time.perf_counter() - t0) / args.steps
+    iso = {p: res[p]["shield"].sum(axis=1)[:, :3].sum(axis=1) / 3.0 for p in res}
+    chi = {p: res[p]["magnet"].sum(axis=0)[:3].sum() / 3.0 for p in res}
+    ctx.close()
+    return {
+        "metric": "property-grid points/s (J + shielding of every nucleus + magnetizability)",
+        "value": round(npts / (ms["mixed"] / 1e3), 1), "unit": "points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms["mixed"], 3),
+        "higher_is_better": True, "dtype": "fp32 (bf16x6 MFMA), fp64 assembly and sums",
+        "data": "synthetic",
+        "config": {"workload": "cage60 (nbf 1800), 60 per-atom product grids of 24 x 12 x 24 points "
+                               "(r <= 6 bohr), shielding of all 60 nuclei and the magnetizability; "
+                               "host arrays in and out", "points_per_step": npts},
+        "fp64_gpu": {"value": round(npts / (ms["fp64"] / 1e3), 1), "ms_per_step": round(ms["fp64"], 3)},
+        "mixed_over_fp64": round(ms["fp64"] / ms["mixed"], 3),
+        "results": {"shielding_iso_ppm_fp64": [round(float(iso["fp64"].min()), 4),
+                                               round(float(iso["fp64"].max()), 4)],
+                    "magnetizability_iso_au_fp64": float(chi["fp64"]),
+                    "shielding_mixed_vs_fp64_max_abs_ppm": float(np.abs(iso["mixed"] - iso["fp64"]).max()),
+                    "magnetizability_mixed_vs_fp64_rel": float(abs(chi["mixed"] - chi["fp64"]) /
+                                                               abs(chi["fp64"]))},
+    }
+
+
 def bench_surrogate(args):
     import torch
     from gimic_amd import synth
@@ -330,7 +390,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference baseline")
     ap.add_argument("--points", type=int, default=0, help="limit points (profiling)")
     ap.add_argument("--workload", default="cage60",
-                    choices=["cage60", "scan", "openshell", "surrogate"])
+                    choices=["cage60", "scan", "openshell", "surrogate", "property"])
     ap.add_argument("--train-steps", type=int, default=12000, help="surrogate training steps")
     ap.add_argument("--sigma", type=float, default=1.0, help="surrogate Fourier frequency scale")
     ap.add_argument("--exclusion", type=float, default=3.0,
@@ -344,7 +404,8 @@ def main():
     from gimic_amd import flux, synth
     assert torch.cuda.is_available(), "no GPU"
     if args.workload != "cage60":
-        fn = {"scan": bench_scan, "openshell": bench_openshell, "surrogate": bench_surrogate}
+        fn = {"scan": bench_scan, "openshell": bench_openshell, "surrogate": bench_surrogate,
+              "property": bench_property}
         print(json.dumps(fn[args.workload](args)))
         return
     dev = torch.device("cuda:0")

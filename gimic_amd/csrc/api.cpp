@@ -119,6 +119,9 @@ struct gimic_ctx {
     // spatial tiles of a chunk: sorted points, their caller indices, sort
     // scratch, and the screening block list of every contraction row tile
     DevBuf spts, perm, bpart, sortscr, bcount, blist;
+    // property quadrature (gimic_ctx_properties): points + weights, tensors,
+    // partition offsets, nuclei, sums
+    DevBuf prop_in, prop_jt, prop_offs, prop_nuc, prop_out;
     // optional per-kernel timing with HIP events on the launch stream:
     // [0] basis evaluation, [1] contraction, [2] finalize
     bool timing = false;
@@ -1068,6 +1071,54 @@ int gimic_ctx_graph_stats(const gimic_ctx *c, long long *builds, long long *laun
         if (!c || !builds || !launches) throw Error(GIMIC_EINVAL, "null argument");
         *builds = c->graph_builds;
         *launches = c->graph_launches;
+    });
+}
+
+int gimic_ctx_properties(gimic_ctx *c, int64_t npts, const double *r, const double *w, int npart,
+                         const int64_t *part_offs, int nnuc, const double *nuclei, int spin,
+                         int precision, double *shield, double *magnet, double *jt) {
+    return guarded([&] {
+        if (!c || !r || !w || !part_offs || !magnet || npts < 0 || npart < 1 || nnuc < 0 ||
+            (nnuc > 0 && (!nuclei || !shield)))
+            throw Error(GIMIC_EINVAL, "null or invalid argument");
+        if (part_offs[0] != 0 || part_offs[npart] != npts)
+            throw Error(GIMIC_EINVAL, "part_offs must run from 0 to npts");
+        if (nnuc >= 65535) throw Error(GIMIC_EINVAL, "at most 65534 nuclei");
+        for (int k = 0; k < npart; k++)
+            if (part_offs[k + 1] < part_offs[k])
+                throw Error(GIMIC_EINVAL, "part_offs must be non-decreasing");
+        hipStream_t st = c->stream;
+        const size_t np = (size_t)std::max<int64_t>(npts, 1);
+        c->prop_in.alloc(np * 32, st, false);  // points then weights
+        c->prop_jt.alloc(np * 72, st, false);
+        c->prop_offs.alloc((size_t)(npart + 1) * 8, st, false);
+        c->prop_nuc.alloc((size_t)std::max(nnuc, 1) * 24, st, false);
+        c->prop_out.alloc((size_t)(nnuc + 1) * npart * 40, st, false);
+        double *d_r = c->prop_in.as<double>(), *d_w = d_r + 3 * np;
+        if (npts > 0) {
+            hip_check(hipMemcpyAsync(d_r, r, (size_t)npts * 24, hipMemcpyHostToDevice, st), "copy r");
+            hip_check(hipMemcpyAsync(d_w, w, (size_t)npts * 8, hipMemcpyHostToDevice, st), "copy w");
+        }
+        hip_check(hipMemcpyAsync(c->prop_offs.p, part_offs, (size_t)(npart + 1) * 8,
+                                 hipMemcpyHostToDevice, st), "copy offs");
+        if (nnuc > 0)
+            hip_check(hipMemcpyAsync(c->prop_nuc.p, nuclei, (size_t)nnuc * 24,
+                                     hipMemcpyHostToDevice, st), "copy nuclei");
+        evaluate(c, npts, d_r, c->prop_jt.as<double>(), {spin}, precision, st, nullptr);
+        hip_check(launch_properties(c->prop_jt.as<double>(), d_r, d_w,
+                                    (const long long *)c->prop_offs.p, c->prop_nuc.as<double>(),
+                                    nnuc, npart, c->prop_out.as<double>(), st),
+                  "properties");
+        std::vector<double> res((size_t)(nnuc + 1) * npart * 5);
+        hip_check(hipMemcpyAsync(res.data(), c->prop_out.p, res.size() * 8, hipMemcpyDeviceToHost,
+                                 st), "copy out");
+        if (jt && npts > 0)
+            hip_check(hipMemcpyAsync(jt, c->prop_jt.p, (size_t)npts * 72, hipMemcpyDeviceToHost,
+                                     st), "copy jt");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        const size_t ns = (size_t)nnuc * npart * 5;
+        if (nnuc > 0) std::memcpy(shield, res.data(), ns * 8);
+        std::memcpy(magnet, res.data() + ns, (size_t)npart * 5 * 8);
     });
 }
 

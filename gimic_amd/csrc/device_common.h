@@ -156,6 +156,12 @@ struct FinalizeArgs {
     double *contrib_acid; // ACID contribution w_p dT^2 or null
 };
 hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s);
+// Shielding (per nucleus) and magnetizability sums per point partition:
+// out[(k * npart + p) * 5 + q], k < nnuc shielding of nucleus k, k == nnuc
+// magnetizability; q = sum w s_x, s_y, s_z, positive part, negative part.
+hipError_t launch_properties(const double *jt, const double *r, const double *w,
+                             const long long *offs, const double *nuc, int nnuc, int npart,
+                             double *out, hipStream_t s);
 // p[0..n) = 0 (a kernel, so it may be captured into a graph)
 hipError_t launch_fill_zero(double *p, size_t n, hipStream_t s);
 

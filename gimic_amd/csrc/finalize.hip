@@ -162,6 +162,70 @@ __global__ __launch_bounds__(256) void plane_reduce_kernel(const double *contrib
     }
 }
 
+// Shielding and magnetizability quadrature on an external grid
+// (jfield.f90:584-929).  Workgroup (p, k): point partition p (the points of
+// one atom's grid, nelpts.info), and nucleus k < nnuc (shielding) or
+// k == nnuc (magnetizability).  With j_b = J (-e_b) the field-direction
+// current (column b of J negated), d = r - R_k and
+// f = -1 / (|d|^3 c^2), c = 137.0359998 (jfield.f90:706):
+//   shielding:       s_b = 1e6 f (d x j_b)_b        (:712-729)
+//   magnetizability: s_b = 1/2 (r x j_b)_b          (:806-822)
+// Per partition: sum w s_x, sum w s_y, sum w s_z, and the positive and
+// negative parts of sum w (s_x + s_y + s_z) (:731-736, :824-829), summed with
+// a fixed-order tree so results are the same on every run.
+__global__ __launch_bounds__(256) void property_kernel(const double *jt, const double *r,
+                                                       const double *w, const long long *offs,
+                                                       const double *nuc, int nnuc, int npart,
+                                                       double *out) {
+    __shared__ double sm[5][256];
+    const int part = blockIdx.x, k = blockIdx.y, t = threadIdx.x;
+    const bool shield = k < nnuc;
+    const double c2 = 137.0359998 * 137.0359998;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long long i = offs[part] + t; i < offs[part + 1]; i += 256) {
+        const double *J = jt + 9 * i;
+        double d[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+        double f = 0.5;
+        if (shield) {
+#pragma unroll
+            for (int e = 0; e < 3; e++) d[e] -= nuc[3 * k + e];
+            const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            f = 1.0e6 * (-1.0 / (d2 * sqrt(d2)) / c2);
+        }
+        double s[3];
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const int b1 = (b + 1) % 3, b2 = (b + 2) % 3;
+            // j = J (-e_b): j_m = -J[m + 3b]
+            s[b] = f * (d[b1] * -J[b2 + 3 * b] - d[b2] * -J[b1 + 3 * b]);
+        }
+        const double pd = s[0] + s[1] + s[2];
+#pragma unroll
+        for (int b = 0; b < 3; b++) acc[b] += w[i] * s[b];
+        if (pd >= 0.0) acc[3] += pd * w[i];
+        else acc[4] += pd * w[i];
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) sm[q][t] = acc[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h)
+#pragma unroll
+            for (int q = 0; q < 5; q++) sm[q][t] += sm[q][t + h];
+        __syncthreads();
+    }
+    if (t < 5) out[((size_t)k * npart + part) * 5 + t] = sm[t][0];
+}
+
+hipError_t launch_properties(const double *jt, const double *r, const double *w,
+                             const long long *offs, const double *nuc, int nnuc, int npart,
+                             double *out, hipStream_t s) {
+    if (npart <= 0) return hipSuccess;
+    hipLaunchKernelGGL(property_kernel, dim3(npart, nnuc + 1), dim3(256), 0, s, jt, r, w, offs, nuc,
+                       nnuc, npart, out);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void fill_zero_kernel(double *p, long long n) {
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
         p[i] = 0.0;

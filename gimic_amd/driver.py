@@ -7,7 +7,10 @@ and runs
                  current flux, per spin case (total; UHF adds alpha, beta,
                  spindens), then ACID when Essential.acid is on
                  -- src/fgimic/gimic.F90:222-261, integral.f90
-  calc=cdens     J B on every grid point (jfield.f90:62-184)
+  calc=cdens     J B on every grid point (jfield.f90:62-184); with
+                 Essential.prop, the shielding of every nucleus and the
+                 magnetizability on the external numgrid grid in coord.au,
+                 gridfile.grd, grid_w.grd and nelpts.info (jfield.f90:584-929)
 printing the reference's stdout blocks.  All integrals come from one tensor
 evaluation per spin case (the reference evaluates J once per integral).
 
@@ -88,6 +91,94 @@ def run_cdens(kw, ctx, g, b, uhf, precision="fp64"):
     return pts, out
 
 
+def _fortran_a(text, w):
+    """Fortran Aw output: right-justified in w columns, truncated on the right"""
+    return text.rjust(w) if len(text) <= w else text[:w]
+
+
+def read_property_files(basedir="."):
+    """numgrid inputs of get_property (jfield.f90:612-657): nucleus
+    coordinates (coord.au), grid points (gridfile.grd) and weights
+    (grid_w.grd), and the points per atom block (second column of
+    nelpts.info); None when any of the first three is missing (the reference
+    then skips the calculation)"""
+    path = lambda f: os.path.join(basedir, f)
+    if not all(os.path.exists(path(f)) for f in ("coord.au", "gridfile.grd", "grid_w.grd")):
+        return None
+    coords = np.loadtxt(path("coord.au"), ndmin=2)[:, :3]
+    pts = np.loadtxt(path("gridfile.grd"), ndmin=2)[:, :3]
+    w = np.loadtxt(path("grid_w.grd"), ndmin=1).reshape(-1)
+    if os.path.exists(path("nelpts.info")):
+        sizes = np.loadtxt(path("nelpts.info"), ndmin=2)[:, 1].astype(np.int64)
+    else:
+        sizes = np.array([len(w)], dtype=np.int64)
+    if len(w) != len(pts) or sizes.sum() != len(w):
+        raise ValueError("gridfile.grd, grid_w.grd and nelpts.info disagree on the point count")
+    return coords, pts, w, sizes
+
+
+def run_property(ctx, basedir=".", spin="total", precision="fp64", out=sys.stdout):
+    """get_property (jfield.f90:584-929): shielding of every nucleus (ppm)
+    and the magnetizability (au and SI) with positive/negative parts and
+    per-atom-block contributions, in the reference's print layout"""
+    files = read_property_files(basedir)
+    if files is None:
+        print(" at least one of the files coord.au, gridfile.grd, and grid_w.grd is missing."
+              "Therefore any property calculation is skipped.", file=out)
+        return None
+    coords, pts, w, sizes = files
+    res = ctx.properties(pts, w, sizes, coords, spin=spin, precision=precision)
+    print(f" npts {len(w):12d}", file=out)
+
+    def contributions(block):
+        # per-block isotropic sums (the reference prints differences of
+        # running totals at the block ends, which are these)
+        print(" atom contributions, total, positive, negative", file=out)
+        csum = np.zeros(3)
+        for l, row in enumerate(block, start=1):
+            v = np.array([row[:3].sum(), row[3], row[4]]) / 3.0
+            print("atom %5d%14.6f%14.6f%14.6f" % (l, *v), file=out)
+            csum += v
+        print(_fortran_a("sum ", 10) + "%14.6f%14.6f%14.6f" % tuple(csum), file=out)
+        print(" " + "*" * 52, file=out)
+
+    for k in range(len(coords)):
+        tot = res["shield"][k].sum(axis=0)
+        print(f" atom {k + 1:12d}", file=out)
+        print(" in ppm", file=out)
+        for b, name in enumerate(("sigma_xx ", "sigma_yy ", "sigma_zz ")):
+            print(" " + _fortran_a(name, 10) + "  %14.6f" % tot[b], file=out)
+        print(_fortran_a("shielding constant    = ", 30) + "  %14.6f" % (tot[:3].sum() / 3.0), file=out)
+        print(_fortran_a("positive contribution = ", 30) + "  %14.6f" % (tot[3] / 3.0), file=out)
+        print(_fortran_a("negative contribution = ", 30) + "  %14.6f" % (tot[4] / 3.0), file=out)
+        print(_fortran_a("sum = ", 30) + "  %14.6f" % ((tot[3] + tot[4]) / 3.0), file=out)
+        print("  ", file=out)
+        contributions(res["shield"][k])
+    print("", file=out)
+    tot = res["magnet"].sum(axis=0)
+    print("", file=out)
+    for b, name in enumerate(("chi_xx ", "chi_yy ", "chi_zz ")):
+        print(" " + _fortran_a(name, 7) + "  %14.8f" % tot[b], file=out)
+    iso = tot[:3].sum() / 3.0
+    print(" in au", file=out)
+    print(" " + _fortran_a("isotropic magnetizability chi = ", 30) + "  %14.6f" % iso, file=out)
+    print(" " + _fortran_a("positive contribution         = ", 30) + "  %14.6f" % (tot[3] / 3.0), file=out)
+    print(" " + _fortran_a("negative contribution         = ", 30) + "  %14.6f" % (tot[4] / 3.0), file=out)
+    print(" " + _fortran_a("sum ", 30) + "  %14.6f" % ((tot[3] + tot[4]) / 3.0), file=out)
+    print("", file=out)
+    si = 7.89104e-29  # J/T^2 per au (jfield.f90:606)
+    print(" in SI units J/T^2 ", file=out)
+    print(" conversion factor: 7.89104*10^-29 J/T^2 ", file=out)
+    print("", file=out)
+    for name, v in (("isotropic magnetizability = ", iso), ("positive contribution     = ", tot[3] / 3.0),
+                    ("negative contribution     = ", tot[4] / 3.0), ("sum ", (tot[3] + tot[4]) / 3.0)):
+        print(_fortran_a(name, 30) + "  %14.6E" % (v * si), file=out)
+    print(" " + "*" * 52, file=out)
+    print("", file=out)
+    contributions(res["magnet"])
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("input", nargs="?", default="gimic.inp")
@@ -101,6 +192,8 @@ def main(argv=None):
         pts, vec = run_cdens(kw, ctx, g, b, uhf, a.precision)
         for spin, v in vec.items():
             np.savetxt(f"jvec_{spin}.txt", np.hstack([pts, v]), fmt="%.10e")
+        if kw["Essential"].get("prop", False):
+            run_property(ctx, os.path.dirname(os.path.abspath(a.input)), precision=a.precision)
     ctx.close()
 
 

@@ -20,7 +20,7 @@ EXPORTS = [
     "gimic_get_gauss_points", "mkgausspoints", "gimic_last_error", "gimic_ctx_create_files",
     "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
     "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_tuning_log", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_tile_stats", "gimic_ctx_tile_stats", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
-    "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_sph_transform", "gimic_quadrature",
+    "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_ctx_properties", "gimic_sph_transform", "gimic_quadrature",
     "gimic_selftest_mfma_layout", "gimic_selftest_mfma32_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_set_stream", "gimic_mlp_info",
     "gimic_mlp_params", "gimic_mlp_loss_grad", "gimic_mlp_train_step", "gimic_mlp_predict",
     "gimic_mlp_predict_unfused",
@@ -85,6 +85,7 @@ def load():
         "gimic_jtensor_batch_device": [P, I64, P, P, I, I, P],
         "gimic_jtensor_multi_device": [P, I64, P, P, I, P, I, P],
         "gimic_flux_planes": [P, I, P, P, P, P, P, P, P, I, I, P, P, P, P],
+        "gimic_ctx_properties": [P, I64, P, P, I, P, I, P, I, I, P, P, P],
         "gimic_sph_transform": [I, P],
         "gimic_quadrature": [D, D, I, I, I, P, P],
         "gimic_selftest_mfma_layout": [P, P],
@@ -323,6 +324,33 @@ class Context:
         check(self._lib.gimic_jtensor_multi_device(self._h, n, ctypes.c_void_p(pts.data_ptr()),
                                                    ctypes.c_void_p(out.data_ptr()), len(spins),
                                                    ptr(sp), PREC[precision], s))
+        return out
+
+    def properties(self, points, weights, part_sizes, nuclei, spin="total", precision="fp64",
+                   return_jt=False):
+        """Shielding and magnetizability quadrature on an external grid
+        (jfield.f90:584-929): points (N,3) with weights (N), split into
+        consecutive partitions of part_sizes points (one per atom grid),
+        nuclei (K,3).  Returns dict(shield=(K, P, 5), magnet=(P, 5)), q =
+        weighted sums of the xx, yy, zz integrands, positive and negative
+        parts of their sum (shielding in ppm, magnetizability in au), and
+        the tensors if return_jt."""
+        r = _f64(points).reshape(-1, 3)
+        w = _f64(weights).reshape(-1)
+        sizes = np.asarray(part_sizes, dtype=np.int64).reshape(-1)
+        offs = np.zeros(len(sizes) + 1, dtype=np.int64)
+        offs[1:] = np.cumsum(sizes)
+        nuc = _f64(nuclei).reshape(-1, 3)
+        shield = np.zeros((len(nuc), len(sizes), 5))
+        magnet = np.zeros((len(sizes), 5))
+        jt = np.zeros((len(r), 9)) if return_jt else None
+        check(self._lib.gimic_ctx_properties(self._h, len(r), ptr(r), ptr(w), len(sizes), ptr(offs),
+                                             len(nuc), ptr(nuc), SPIN[spin], PREC[precision],
+                                             ptr(shield), ptr(magnet),
+                                             ptr(jt) if jt is not None else None))
+        out = dict(shield=shield, magnet=magnet)
+        if return_jt:
+            out["jt"] = jt
         return out
 
     def flux_planes(self, planes, spin="total", precision="fp64", return_jt=False,

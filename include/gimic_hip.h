@@ -201,6 +201,24 @@ int gimic_flux_planes(gimic_ctx *ctx, int nplanes, const int64_t *plane_offs, co
                       const double *centers, const double *radii, int spin, int precision,
                       double *out, double *out_mod, double *out_acid, double *jt);
 
+/* Magnetic properties on an external quadrature grid (jfield.f90:584-929,
+ * Essential.prop with numgrid files).  J at npts points r[3*i..] with
+ * weights w[i]; the points fall into npart consecutive partitions (the
+ * per-atom blocks of nelpts.info), partition p = [part_offs[p],
+ * part_offs[p+1]).  For each nucleus k (nuclei[3*k..], coord.au) and
+ * partition p, shield[(k*npart + p)*5 + q] holds, with d = r - R_k,
+ * f = -1e6 / (|d|^3 c^2), c = 137.0359998, and j_b = -J e_b:
+ *   q = 0, 1, 2   sum w f (d x j_b)_b for b = x, y, z (shielding, ppm)
+ *   q = 3, 4      positive and negative parts of sum w f sum_b (d x j_b)_b
+ * and magnet[p*5 + q] the same sums of 1/2 (r x j_b)_b (magnetizability,
+ * au).  The reference prints totals over all partitions, the isotropic
+ * value (sum of the three) / 3, and per-atom contributions; the library
+ * sums each partition with a fixed-order tree.  Host pointers; jt
+ * (optional) receives the tensors. */
+int gimic_ctx_properties(gimic_ctx *ctx, int64_t npts, const double *r, const double *w,
+                         int npart, const int64_t *part_offs, int nnuc, const double *nuclei,
+                         int spin, int precision, double *shield, double *magnet, double *jt);
+
 /* Real solid-harmonic transform for angular momentum l <= 5:
  * T[(m+l)*ncart + c], m = -l..l, cartesian components in the reference's
  * non-TURBOMOLE order (gtodefs.f90:86-106); for l <= 1 the identity (s, and
