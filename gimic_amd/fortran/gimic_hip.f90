@@ -111,6 +111,26 @@ module gimic_hip
             type(c_ptr), value :: out_mod, out_acid, jt
             integer(c_int) :: rc
         end function
+        ! get_property (jfield.f90:584-929): shielding sums per nucleus and atom
+        ! block, magnetizability sums per atom block, 5 values each
+        function gimic_ctx_properties(ctx, npts, r, w, npart, part_offs, nnuc, nuclei, &
+                spin, precision, shield, magnet, jt) &
+                bind(c, name='gimic_ctx_properties') result(rc)
+            import :: c_ptr, c_int, c_int64_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: npts
+            real(c_double), dimension(3, *), intent(in) :: r
+            real(c_double), dimension(*), intent(in) :: w
+            integer(c_int), value :: npart
+            integer(c_int64_t), dimension(*), intent(in) :: part_offs
+            integer(c_int), value :: nnuc
+            real(c_double), dimension(3, *), intent(in) :: nuclei
+            integer(c_int), value :: spin, precision
+            real(c_double), dimension(5, *), intent(out) :: shield
+            real(c_double), dimension(5, *), intent(out) :: magnet
+            type(c_ptr), value :: jt
+            integer(c_int) :: rc
+        end function
         function gimic_ctx_destroy(ctx) bind(c, name='gimic_ctx_destroy') result(rc)
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx
