@@ -118,7 +118,7 @@ struct gimic_ctx {
     DevBuf blk_beg, blk_sh;
     // spatial tiles of a chunk: sorted points, their caller indices, sort
     // scratch, and the screening block list of every contraction row tile
-    DevBuf spts, perm, bpart, sortscr, bcount, blist;
+    DevBuf spts, perm, bpart, sortscr, bcount, blist, toff;
     // property quadrature (gimic_ctx_properties): points + weights, tensors,
     // partition offsets, nuclei, sums
     DevBuf prop_in, prop_jt, prop_offs, prop_nuc, prop_out;
@@ -502,6 +502,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
     grow(c->sortscr, (size_t)sort_scratch_ints((int)rows_cap) * 4, false);
     grow(c->bcount, rows_cap / 128 * 4, false);
     grow(c->blist, rows_cap / 128 * nblk * 4, false);
+    grow(c->toff, (rows_cap / 128 + 1) * 4, false);
     const int nmatset = M.nmatset;
     for (size_t start = 0; start < (size_t)npts; start += chunk) {
         const int n_c = (int)std::min(chunk, (size_t)npts - start);
@@ -600,17 +601,20 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             else if (variant == 2) plan(contract_mx_rows(), contract_mx_cols());
             else plan(kWgRows, kNT);
         };
-        // pre-split kernels add row-major partial sums into a cleared buffer
-        auto row_major = [](int variant) { return (variant >= 3 && variant <= 5) || variant >= 8; };
+        // pre-split 16x16 kernels: row-major partial sums added into a cleared
+        // buffer, and basis rows compressed to each row tile's listed blocks
+        // (lists and offsets built before the basis evaluation)
+        auto presplit = [](int variant) { return (variant >= 3 && variant <= 5) || variant >= 8; };
+        auto tile_rows_of = [&](int variant) {
+            return variant == 5 || variant >= 8 ? 256 : contract_mx_rows();
+        };
         auto run = [&](int variant, int split = -1) {
             plan_variant(variant, split);
-            if (row_major(variant))
+            if (presplit(variant))
                 hip_check(launch_fill_zero(ca.part, (size_t)nmatset * ca.nsplit * kNQ * rows, st),
                           "fill_zero");
-            // kernels that walk block lists: FP64 and pre-split 16x16 (128 or 256 rows)
-            if (variant == 0 || (variant >= 3 && variant <= 5) || variant >= 8)
-                lists(variant == 0 ? kWgRows
-                      : variant == 5 || variant >= 8 ? 256 : contract_mx_rows());
+            if (variant == 0) lists(kWgRows);  // the FP64 kernel walks lists over dense rows
+            ca.toff = presplit(variant) ? c->toff.as<int>() : nullptr;
             if (variant == 0) hip_check(launch_contract_f64(ca, st), "contract_f64");
             else if (variant == 1) hip_check(launch_contract_bf16x6(ca, st), "contract_bf16x6");
             else if (variant == 2) hip_check(launch_contract_mx(ca, st), "contract_mx");
@@ -624,6 +628,18 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         // basis evaluation output each contraction kernel reads
         auto eval_mode = [](int variant) { return variant == 0 ? 0 : variant >= 3 ? 2 : 1; };
         auto eval = [&](int variant) {
+            ea.toff = nullptr;
+            if (presplit(variant)) {
+                const int tr = tile_rows_of(variant);
+                lists(tr);
+                hip_check(launch_tile_offsets(c->bcount.as<int>(), rows / tr, c->toff.as<int>(), st),
+                          "tile_offsets");
+                ea.bcount = c->bcount.as<int>();
+                ea.blist = c->blist.as<int>();
+                ea.toff = c->toff.as<int>();
+                ea.nblk = nblk;
+                ea.tile_rows = tr;
+            }
             hip_check(launch_basis_eval(ea, eval_mode(variant), st), "basis_eval");
         };
         const bool tune = prec == GIMIC_PREC_MIXED && c->mixed_variant < 0;
@@ -693,7 +709,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         FinalizeArgs fa{};
         fa.part = c->part.as<double>();
         fa.nsplit = nsplit;
-        fa.part_rows = row_major(variant) ? 1 : 0;
+        fa.part_rows = presplit(variant) ? 1 : 0;
         fa.rows = rows;
         fa.npts = n_c;
         fa.nmatset = nmatset;
@@ -977,7 +993,7 @@ int gimic_flux_planes(gimic_ctx *c, int nplanes, const int64_t *offs, const doub
                 (uintptr_t)M.b32.p, (uintptr_t)c->phis.p, (uintptr_t)M.bs3.p,
                 (uintptr_t)c->mixed_split, (uintptr_t)c->spts.p, (uintptr_t)c->perm.p,
                 (uintptr_t)c->bpart.p, (uintptr_t)c->sortscr.p, (uintptr_t)c->bcount.p,
-                (uintptr_t)c->blist.p};
+                (uintptr_t)c->blist.p, (uintptr_t)c->toff.p};
             if (!c->graph_exec || key != c->graph_key) {
                 if (c->graph_exec) {
                     hip_check(hipGraphExecDestroy(c->graph_exec), "hipGraphExecDestroy");

@@ -25,6 +25,34 @@ namespace gimic {
 
 __constant__ double c_sph[466];  // T_l, l = 2..5, (2l+1) x ncart(l) row-major
 
+// Radial parts g = sum c e^{-a s} and h = sum a c e^{-a s} of one
+// contraction.  The primitives' exponents and coefficients are loaded in
+// groups of four before any exponential, so a group costs one memory
+// latency rather than one per primitive (the terms are summed in the same
+// order as a plain loop).
+__device__ __forceinline__ void radial_parts(const EvalArgs &a, const DevShell &sh, double s,
+                                             double &g, double &h) {
+    g = 0.0;
+    h = 0.0;
+    for (int p0 = 0; p0 < sh.nprim; p0 += 4) {
+        double xp[4], cc[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool in = p0 + k < sh.nprim;
+            xp[k] = in ? a.exps[sh.poff + p0 + k] : 0.0;
+            cc[k] = in ? a.ncc[sh.poff + p0 + k] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (p0 + k < sh.nprim) {
+                const double q = cc[k] * exp(-xp[k] * s);
+                g += q;
+                h += xp[k] * q;
+            }
+        }
+    }
+}
+
 // LDS image of a tile of rows for basis_eval_staged_kernel:
 //   [tile_points][lda/8][3][8] BF16 parts of phi, then [3][tile_points][lda] FP32 gradients
 extern __shared__ __align__(16) unsigned char basis_tile[];
@@ -219,14 +247,7 @@ __global__ __launch_bounds__(256) void basis_eval_kernel(EvalArgs a) {
         rr[2] = r[2] - R[2];
         const double s = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
         zero = sqrt(s) > sh.radius;
-        if (!zero) {
-            for (int p = 0; p < sh.nprim; p++) {
-                const double xp = a.exps[sh.poff + p];
-                const double q = a.ncc[sh.poff + p] * exp(-xp * s);
-                g += q;
-                h += xp * q;
-            }
-        }
+        if (!zero) radial_parts(a, sh, s, g, h);
     }
     if (zero) {
         for (int f = 0; f < sh.ncomp_out; f++) store<MODE>(a, row, sh.fout + f, 0.0, 0.0, 0.0, 0.0);
@@ -243,55 +264,93 @@ __global__ __launch_bounds__(256) void basis_eval_kernel(EvalArgs a) {
 
 // Split output through LDS (see the file comment).  Grid: rows / tile_points
 // workgroups; tile_points divides 128 and the LDS image is
-// tile_points * lda * 18 bytes.
+// tile_points * lda * 18 bytes (then, compressed, nblk shorts).
+//
+// Compressed output (a.toff non-null): the workgroup's rows belong to row
+// tile t of a.tile_rows rows, which keeps only its listed blocks.  A shell
+// is evaluated only if its first block is listed: a shell with a function
+// in an unlisted block lies beyond its radius for every point of the tile
+// (which is what leaving the block out means), and a shell that is not has
+// all its functions in listed blocks, adjacent in the list, so its
+// functions stay contiguous in the compressed columns.
 template <int LMAX>
 __global__ __launch_bounds__(256) void basis_eval_staged_kernel(EvalArgs a) {
-    const int tid = threadIdx.x, tp = a.tile_points, lda = a.lda;
+    const int tid = threadIdx.x, tp = a.tile_points;
     const size_t row0 = (size_t)blockIdx.x * tp;
-    const size_t phi_bytes = (size_t)tp * lda * 6, grad_bytes = (size_t)tp * lda * 12;
-    {
-        uint4 *z = (uint4 *)basis_tile;
-        const size_t n = (phi_bytes + grad_bytes) / 16;
-        for (size_t i = tid; i < n; i += 256) z[i] = make_uint4(0u, 0u, 0u, 0u);
+    int wt = a.lda;  // functions per row of this tile's output
+    short *bpos = nullptr;
+    __bf16 *phi_dst = a.phis + row0 * 3 * (size_t)a.lda;
+    float *grad_dst = a.grad32 + row0 * (size_t)a.lda;
+    size_t grad_plane = (size_t)a.gplane;
+    if (a.toff) {
+        const int t = (int)(row0 / a.tile_rows);
+        const int na = a.bcount[t];
+        if (na == 0) return;  // nothing of this tile is read (uniform per workgroup)
+        wt = 32 * na;
+        bpos = (short *)(basis_tile + a.lds_image);
+        for (int i = tid; i < a.nblk; i += 256) bpos[i] = -1;
+        __syncthreads();
+        for (int i = tid; i < na; i += 256) bpos[a.blist[(size_t)t * a.nblk + i]] = (short)i;
+        const size_t base = (size_t)a.toff[t] * a.tile_rows * 32;
+        const size_t rin = row0 - (size_t)t * a.tile_rows;
+        phi_dst = a.phis + base * 3 + rin * 3 * (size_t)wt;
+        grad_dst = a.grad32 + base * 3 + rin * (size_t)wt;
+        grad_plane = (size_t)a.tile_rows * wt;
     }
-    __syncthreads();
-    const int items = tp * a.nshell;
-    for (int it = tid; it < items; it += 256) {
-        const int lp = it / a.nshell;
-        const size_t row = row0 + lp;
-        if (row >= (size_t)a.npts) continue;  // padding rows stay zero
-        const DevShell sh = a.shells[it % a.nshell];
-        const double *r = a.pts + 3 * row;
-        const double *R = a.coords + 3 * sh.atom;
-        double rr[3] = {r[0] - R[0], r[1] - R[1], r[2] - R[2]};
-        const double s = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
-        if (sqrt(s) > sh.radius) continue;  // screened: zeros already in place
-        double g = 0.0, h = 0.0;
-        for (int p = 0; p < sh.nprim; p++) {
-            const double xp = a.exps[sh.poff + p];
-            const double q = a.ncc[sh.poff + p] * exp(-xp * s);
-            g += q;
-            h += xp * q;
+    // rows per pass: as many as the LDS image holds at this width (at least
+    // one: the image holds a dense row)
+    const int rpp = max(1, min(tp, (int)(a.lds_image / ((size_t)wt * 18))));
+    for (int r0 = 0; r0 < tp; r0 += rpp) {
+        const int nr = min(rpp, tp - r0);
+        EvalArgs b = a;
+        b.lda = wt;         // row stride of the LDS image
+        b.tile_points = nr; // rows in the image
+        const size_t phi_bytes = (size_t)nr * wt * 6, grad_bytes = (size_t)nr * wt * 12;
+        {
+            uint4 *z = (uint4 *)basis_tile;
+            const size_t n = (phi_bytes + grad_bytes) / 16;
+            for (size_t i = tid; i < n; i += 256) z[i] = make_uint4(0u, 0u, 0u, 0u);
         }
-        if (a.turbomole) {
-            if (a.spherical) dispatch_l<3, true, true, LMAX>(a, lp, sh, rr, g, h);
-            else dispatch_l<3, true, false, LMAX>(a, lp, sh, rr, g, h);
-        } else {
-            if (a.spherical) dispatch_l<3, false, true, LMAX>(a, lp, sh, rr, g, h);
-            else dispatch_l<3, false, false, LMAX>(a, lp, sh, rr, g, h);
+        __syncthreads();
+        const int items = nr * a.nshell;
+        for (int it = tid; it < items; it += 256) {
+            const int lp = it / a.nshell;
+            const size_t row = row0 + r0 + lp;
+            if (row >= (size_t)a.npts) continue;  // padding rows stay zero
+            DevShell sh = a.shells[it % a.nshell];
+            if (bpos) {
+                const int cb = bpos[sh.fout >> 5];
+                if (cb < 0) continue;  // zero over the whole tile
+                sh.fout = cb * 32 + (sh.fout & 31);
+            }
+            const double *r = a.pts + 3 * row;
+            const double *R = a.coords + 3 * sh.atom;
+            double rr[3] = {r[0] - R[0], r[1] - R[1], r[2] - R[2]};
+            const double s = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
+            if (sqrt(s) > sh.radius) continue;  // screened: zeros already in place
+            double g, h;
+            radial_parts(a, sh, s, g, h);
+            if (a.turbomole) {
+                if (a.spherical) dispatch_l<3, true, true, LMAX>(b, lp, sh, rr, g, h);
+                else dispatch_l<3, true, false, LMAX>(b, lp, sh, rr, g, h);
+            } else {
+                if (a.spherical) dispatch_l<3, false, true, LMAX>(b, lp, sh, rr, g, h);
+                else dispatch_l<3, false, false, LMAX>(b, lp, sh, rr, g, h);
+            }
         }
-    }
-    __syncthreads();
-    // rows row0 .. row0 + tp - 1 are contiguous in both outputs
-    {
-        const uint4 *src = (const uint4 *)basis_tile;
-        uint4 *dst = (uint4 *)(a.phis + row0 * 3 * (size_t)lda);
-        for (size_t i = tid; i < phi_bytes / 16; i += 256) dst[i] = src[i];
-    }
-    for (int d = 0; d < 3; d++) {
-        const uint4 *src = (const uint4 *)(basis_tile + phi_bytes + (size_t)d * tp * lda * 4);
-        uint4 *dst = (uint4 *)(a.grad32 + (size_t)d * a.gplane + row0 * lda);
-        for (size_t i = tid; i < (size_t)tp * lda / 4; i += 256) dst[i] = src[i];
+        __syncthreads();
+        // this pass's rows are contiguous in both outputs
+        {
+            const uint4 *src = (const uint4 *)basis_tile;
+            uint4 *dst = (uint4 *)(phi_dst + (size_t)r0 * 3 * wt);
+            for (size_t i = tid; i < phi_bytes / 16; i += 256) dst[i] = src[i];
+        }
+        for (int d = 0; d < 3; d++) {
+            const uint4 *src = (const uint4 *)(basis_tile + phi_bytes + (size_t)d * nr * wt * 4);
+            uint4 *dst = (uint4 *)(grad_dst + d * grad_plane + (size_t)r0 * wt);
+            for (size_t i = tid; i < (size_t)nr * wt / 4; i += 256) dst[i] = src[i];
+        }
+        __syncthreads();  // the image is read out before the next pass clears it
     }
 }
 
@@ -310,16 +369,27 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
     else if (mode == 1)
         hipLaunchKernelGGL(basis_eval_kernel<1>, dim3(nblk), dim3(256), 0, s, a);
     else {
-        // largest tile of 1, 2, 4 or 8 rows whose LDS image fits in 72 KiB
-        // (two workgroups per CU)
+        // dense: the largest tile of 1, 2, 4 or 8 rows whose LDS image fits in
+        // 72 KiB (two workgroups per CU).  Compressed: 8 rows per workgroup
+        // staged in passes through an image of 36 KiB or one dense row,
+        // whichever is larger (four workgroups per CU at nbf <= 2048)
         EvalArgs b = a;
         const size_t row_bytes = (size_t)a.lda * 18;
         int tp = 8;
-        while (tp > 1 && tp * row_bytes > 72 * 1024) tp >>= 1;
-        if (row_bytes > 160 * 1024) return hipErrorInvalidValue;
+        size_t image;
+        if (a.toff) {
+            image = std::max<size_t>(36 * 1024, row_bytes);
+        } else {
+            while (tp > 1 && tp * row_bytes > 72 * 1024) tp >>= 1;
+            image = tp * row_bytes;
+        }
+        if (row_bytes > 150 * 1024) return hipErrorInvalidValue;
         b.tile_points = tp;
-        const size_t lds = tp * row_bytes;
+        b.lds_image = image;
+        const size_t lds = image + (a.toff ? (size_t)a.nblk * 2 : 0);
         if (a.rows % tp || a.lmax < 0 || a.lmax > kDevMaxL) return hipErrorInvalidValue;
+        if (a.toff && (a.tile_rows % tp || a.rows % a.tile_rows || a.nblk * 32 != a.lda))
+            return hipErrorInvalidValue;
         auto launch = [&](auto kernel) -> hipError_t {
             hipError_t e = hipFuncSetAttribute((const void *)kernel,
                                                hipFuncAttributeMaxDynamicSharedMemorySize,

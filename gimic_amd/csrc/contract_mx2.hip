@@ -147,8 +147,17 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
     const int lda = a.lda;
     const int nks = lda / 32;
     const size_t rows = (size_t)a.rows;
-    const size_t gplane = (size_t)a.gplane;
-    const size_t arow = 3 * (size_t)lda;  // BF16 per split row
+    const int *L = a.blist + (size_t)rtile * a.nblk;
+    const int na = a.bcount[rtile];  // listed blocks
+    // This tile's basis rows: dense [row][lda], or compressed to the listed
+    // blocks (a.toff non-null: the basis kernel wrote only those, so A and
+    // the gradients are addressed by list position)
+    const bool cmp = a.toff != nullptr;
+    const size_t wcols = cmp ? (size_t)kNu * na : (size_t)lda;  // functions per row
+    const size_t arow = 3 * wcols;                              // BF16 per split row
+    const __bf16 *Atile = cmp ? a.as3 + (size_t)a.toff[rtile] * kRows * 96 : a.as3 + row0 * arow;
+    const float *Gtile = cmp ? a.g32 + (size_t)a.toff[rtile] * kRows * 96 : a.g32 + row0 * wcols;
+    const size_t gpl = cmp ? (size_t)kRows * wcols : (size_t)a.gplane;  // gradient plane stride
     const int np = pieces_of(wave, WAVES);  // pieces this wavefront copies per block
     // this workgroup's partial sums, row-major [row][kNQ]
     double *pout = a.part + ((size_t)set * a.nsplit + split) * kNQ * rows;
@@ -162,10 +171,8 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
     }
 
     // lane (g, c) of row block rb: row wave*32 + rb*16 + c, k = 32 ks + 8 g .. + 7
-    const __bf16 *Aw = a.as3 + (row0 + wave * 16 * kRB + c) * arow + g * 24;
+    const __bf16 *Aw = Atile + (size_t)(wave * 16 * kRB + c) * arow + g * 24;
     const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kBlk;
-    const int *L = a.blist + (size_t)rtile * a.nblk;
-    const int na = a.bcount[rtile];  // listed blocks
     const int e0 = (int)((long long)split * na / a.nsplit);
     const int e1 = (int)((long long)(split + 1) * na / a.nsplit);
 
@@ -186,7 +193,7 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
         for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
             for (int p = 0; p < 3; p++)
-                an[rb][p] = *(const v8bf *)(Aw + rb * 16 * arow + (size_t)L[0] * 96 + p * 8);
+                an[rb][p] = *(const v8bf *)(Aw + rb * 16 * arow + (size_t)(cmp ? 0 : L[0]) * 96 + p * 8);
         __builtin_amdgcn_s_waitcnt(kWaitAll);
         __syncthreads();
 
@@ -243,7 +250,8 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
                 for (int j = 2 * cb; j < 2 * cb + 2; j++)
                     if (more && j < kALoads)
                         an[j / 3][j % 3] = *(const v8bf *)(Aw + (j / 3) * 16 * arow +
-                                                          (size_t)a_blk * 96 + (j % 3) * 8);
+                                                          (size_t)(cmp ? ks + 1 : a_blk) * 96 +
+                                                          (j % 3) * 8);
                 if (copy && cb >= kCopyFrom && cb - kCopyFrom < np)
                     stage_piece<WAVES>(cblk, cbuf, wave, lane, cb - kCopyFrom);
                 __builtin_amdgcn_sched_barrier(0);
@@ -288,16 +296,17 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
         __bf16 pc[2][3], pn[2][3];
         float Gc[2][3], Gn[2][3];
         auto load_row = [&](int it, __bf16 (&pv)[2][3], float (&Gv)[2][3]) {
-            const size_t p = row0 + row_of(it);
+            const size_t rl = row_of(it);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const int col = nt * kNu + 16 * h + c;
-                const __bf16 *pe = a.as3 + p * arow + (col >> 3) * 24 + (col & 7);
+                // column of function nt*32 + 16h + c in this tile's basis rows
+                const int acol = (cmp ? e : nt) * kNu + 16 * h + c;
+                const __bf16 *pe = Atile + rl * arow + (acol >> 3) * 24 + (acol & 7);
 #pragma unroll
                 for (int k = 0; k < 3; k++) pv[h][k] = pe[8 * k];
-                const size_t e = p * lda + col;
+                const size_t gi = rl * wcols + acol;
 #pragma unroll
-                for (int d = 0; d < 3; d++) Gv[h][d] = a.g32[d * gplane + e];
+                for (int d = 0; d < 3; d++) Gv[h][d] = Gtile[d * gpl + gi];
             }
         };
         load_row(0, pc, Gc);

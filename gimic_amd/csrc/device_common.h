@@ -96,7 +96,14 @@ struct EvalArgs {
     // three BF16 parts x = h + m + l (exact), layout [row][lda/8][3][8]
     __bf16 *phis;
     int tile_points;     // points per workgroup of the LDS-staged kernel (set by the launcher)
+    size_t lds_image;    // bytes of its LDS image (set by the launcher)
     int lmax;            // largest shell l of the basis
+    // Compressed split output (staged kernel, toff non-null): a row tile of
+    // tile_rows rows keeps only its bcount[t] listed 32-function blocks,
+    // blist[t * nblk ...]; its rows start at element toff[t] * tile_rows * 96
+    // of phis ([row][32 n / 8][3][8]) and of grad32 ([3][tile_rows][32 n])
+    const int *bcount, *blist, *toff;
+    int nblk, tile_rows;
 };
 hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s);
 hipError_t upload_basis_tables(const double *sph, int nsph, hipStream_t s);
@@ -120,6 +127,8 @@ struct ContractArgs {
     // a tile takes the listed N-blocks [s * count / nsplit, (s+1) * count / nsplit)
     const int *bcount, *blist;
     int nblk;
+    // compressed operands (pre-split kernels; see EvalArgs): null for dense
+    const int *toff;
 };
 hipError_t launch_contract_f64(const ContractArgs &a, hipStream_t s);
 hipError_t launch_contract_bf16x6(const ContractArgs &a, hipStream_t s);
@@ -188,6 +197,8 @@ struct TileArgs {
     int *bcount, *blist;  // [rows / tile_rows], [rows / tile_rows][nblk]
 };
 hipError_t launch_block_lists(const TileArgs &a, hipStream_t s);
+// toff[t] = sum of bcount[0 .. t-1] for t = 0 .. ntiles (one workgroup)
+hipError_t launch_tile_offsets(const int *bcount, int ntiles, int *toff, hipStream_t s);
 
 hipError_t run_mfma_layout_probe(double *h_f64, float *h_bf16);
 hipError_t run_mfma32_layout_probe(float *h_out);  // 2048 floats

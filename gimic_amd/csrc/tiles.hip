@@ -254,7 +254,37 @@ __global__ __launch_bounds__(256) void block_list_kernel(TileArgs a) {
     if (t == 0) a.bcount[blockIdx.x] = base;
 }
 
+// exclusive prefix sum of the tiles' block counts, toff[0 .. n]
+__global__ __launch_bounds__(256) void tile_offsets_kernel(const int *cnt, int n, int *toff) {
+    __shared__ int s[256];
+    const int t = threadIdx.x;
+    const int per = (n + 255) / 256;
+    const int beg = min(n, t * per), end = min(n, beg + per);
+    int sum = 0;
+    for (int i = beg; i < end; i++) sum += cnt[i];
+    s[t] = sum;
+    __syncthreads();
+    for (int w = 1; w < 256; w <<= 1) {
+        const int x = t >= w ? s[t - w] : 0;
+        __syncthreads();
+        s[t] += x;
+        __syncthreads();
+    }
+    int run = s[t] - sum;
+    for (int i = beg; i < end; i++) {
+        toff[i] = run;
+        run += cnt[i];
+    }
+    if (t == 255) toff[n] = s[255];
+}
+
 }  // namespace
+
+hipError_t launch_tile_offsets(const int *bcount, int ntiles, int *toff, hipStream_t s) {
+    if (ntiles <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(tile_offsets_kernel, dim3(1), dim3(256), 0, s, bcount, ntiles, toff);
+    return hipGetLastError();
+}
 
 int sort_scratch_ints(int n) {
     const int nb = (n + kSortBlock - 1) / kSortBlock;
