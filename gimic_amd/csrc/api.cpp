@@ -472,13 +472,13 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
 
     const size_t ld = (size_t)c->lda;
     // chunk of points per pass: basis values + gradients of one chunk stay
-    // within ~8 GiB (FP64: 32 bytes per point and function)
+    // within ~24 GiB (FP64: 32 bytes per point and function)
     // (mixed: FP32 values 4 + split values 6 + FP32 gradients 12 bytes)
     const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 22);
     // row granularity: a multiple of every contraction kernel's workgroup
     // height (128; 256 for the 256-row pre-split kernel)
     const size_t gran = prec == GIMIC_PREC_FP64 ? (size_t)kWgRows : 256;
-    size_t cap = (size_t)8 << 30;
+    size_t cap = (size_t)24 << 30;
     size_t chunk = std::max<size_t>(gran, (cap / per_row) / gran * gran);
     chunk = std::min(chunk, round_up((size_t)npts, gran));
     const size_t rows_cap = chunk;
