@@ -432,6 +432,10 @@ def main():
                       7: "contract_mx3p_kernel", 8: "contract_mx2_kernel<2,2,8>",
                       9: "contract_mx2_kernel<3,2,8>"}.get(variant, "?")
     roof["split"] = split
+    # the mixed kernels issue six BF16 MFMA products per algorithmic product
+    # (FP32 operands split into three BF16 parts): matrix-pipe use is 6x frac
+    roof["mfma_issued_tflops"] = round(6.0 * roof["achieved"], 1)
+    roof["mfma_issued_frac"] = round(6.0 * roof["achieved"] / roof["peak"], 4)
     roof["traffic"] = measured_traffic(roof["kernel"], split, roof["points_per_launch"])
     if roof["traffic"] is not None:
         roof["traffic_note"] = ("L2-miss bytes per launch (Infinity Cache + HBM), rocprofv3 FETCH_SIZE/"
