@@ -172,36 +172,51 @@ def scan_planes(system, nplanes=256, seed=7):
               "fixcoord": [0.0, 0.0, 10.0], "distance": half + float(off),
               "height": [-8.0, 8.0], "width": [-10.0, 10.0], "grid_points": [n1, n2, 0]}
         g = gridmod.new_grid(kw, c)
-        planes.append(flux.plane_of(g, gridmod.get_magnet(g, "X")))
+        planes.append((g, gridmod.get_magnet(g, "X")))
     return planes
 
 
 def bench_scan(args):
+    """A scan step is one flux_plane_grids call: the device generates every
+    plane's points and weights from its 1-D rules, evaluates the tensors and
+    sums the flux, in one hipGraph launch.  The same planes passed as host
+    point arrays (flux_planes) are timed beside it."""
     import torch
-    from gimic_amd import synth
+    from gimic_amd import flux, synth
     system = synth.sheet30()
     ctx = system.context(screening=1e-8)
-    planes = scan_planes(system)
+    scan = scan_planes(system)
+    grids, fields = [g for g, _ in scan], [b for _, b in scan]
+    planes = [flux.plane_of(g, b) for g, b in scan]
     npts = sum(len(p["weights"]) for p in planes)
     res = {}
     for prec in ("mixed", "fp64"):
         out = {}
-        for mode in ("eager", "graph"):
-            ctx.set_graph(mode == "graph")
+        for mode in ("eager", "graph", "graph_host_arrays"):
+            ctx.set_graph(mode != "eager")
+            if mode == "graph_host_arrays":
+                call = lambda: ctx.flux_planes(planes, precision=prec, modulus=True)
+            else:
+                call = lambda: ctx.flux_plane_grids(grids, fields, precision=prec, modulus=True)
             for _ in range(args.warmup):
-                ctx.flux_planes(planes, precision=prec, modulus=True)
+                call()
             t0 = time.perf_counter()
             for _ in range(args.steps):
-                f, m = ctx.flux_planes(planes, precision=prec, modulus=True)
+                f, m = call()
             out[mode] = (time.perf_counter() - t0) / args.steps
-        res[prec] = dict(flux=f, ms=out)
+            if mode == "graph":
+                fg = f
+            elif mode == "graph_host_arrays":
+                assert np.array_equal(f, fg), "device-generated and host planes differ"
+        res[prec] = dict(flux=fg, ms=out)
     ctx.close()
     f64, fmx = res["fp64"]["flux"], res["mixed"]["flux"]
     rel = np.abs(fmx[:, 0] - f64[:, 0]) / np.maximum(np.abs(f64[:, 0]), 1e-300)
     big = np.abs(f64[:, 0]) >= 1e-3 * np.abs(f64[:, 0]).max()
     t = res["mixed"]["ms"]["graph"]
     return {
-        "metric": "plane-scan J-tensor points/s (one hipGraph launch per scan)",
+        "metric": "plane-scan J-tensor points/s (planes generated on the device, one hipGraph "
+                  "launch per scan)",
         "value": round(npts / t, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * t, 3), "higher_is_better": True,
         "dtype": "fp32 (bf16x6 MFMA), fp64 assembly", "data": "synthetic",
@@ -447,6 +462,12 @@ def main():
 
     nb = np.linalg.norm(J_f64_sample, axis=1)
     e = np.linalg.norm(J_mixed_sample - J_f64_sample, axis=1) / np.maximum(nb, 1e-3 * nb.max())
+    log("host buffers")
+    # PCIe-inclusive rate: host points in, host tensors out through
+    # gimic_jtensor_batch (one call; the headline keeps both in HBM)
+    t0 = time.perf_counter()
+    ctx.jtensor(pts, precision="mixed")
+    host_s = time.perf_counter() - t0
     log("plane flux")
     g, B = plane_grid(system)
     f64 = np.array(flux.integrate_current(ctx, g, B, precision="fp64"))
@@ -482,6 +503,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "wall_s": {"mixed": round(wall_mixed, 3), "fp64": round(wall_f64, 3)},
+        "host_buffers": {"points_per_s": round(npts / host_s, 1), "ms": round(1e3 * host_s, 3),
+                         "note": "mixed, host points in and host tensors out (PCIe-inclusive)"},
     }
     print(json.dumps(res))
 

@@ -122,6 +122,8 @@ struct gimic_ctx {
     // property quadrature (gimic_ctx_properties): points + weights, tensors,
     // partition offsets, nuclei, sums
     DevBuf prop_in, prop_jt, prop_offs, prop_nuc, prop_out;
+    // plane-grid descriptors (gimic_flux_plane_grids)
+    DevBuf gspec;
     // optional per-kernel timing with HIP events on the launch stream:
     // [0] basis evaluation, [1] contraction, [2] finalize
     bool timing = false;
@@ -911,6 +913,140 @@ int gimic_jtensor_batch(gimic_ctx *c, int64_t npts, const double *r, double *jt,
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Work buffers of a plane batch: points then weights (pts_tmp), tensors,
+// per-point contributions (current, signed modulus, ACID: every point is
+// written by finalize_kernel, so no clearing), plane index, plane
+// parameters, offsets and per-plane sums.
+void flux_buffers(gimic_ctx *c, int64_t npts, int nplanes) {
+    hipStream_t st = c->stream;
+    const size_t np = (size_t)std::max<int64_t>(npts, 1);
+    c->pts_tmp.alloc(np * 24 + np * 8, st, false);
+    c->jt_tmp.alloc(np * 72, st, false);
+    c->contrib.alloc(np * 24, st, false);
+    c->planes.alloc(np * 4, st, false);
+    c->pparams.alloc((size_t)nplanes * 80, st, false);
+    c->offs.alloc((size_t)(nplanes + 1) * 8, st, false);
+    c->outs.alloc((size_t)nplanes * 72, st, false);
+}
+
+// Plane flux over the points, weights and plane indices that `prepare(st)`
+// leaves in the work buffers (a device launch; in graph mode it is captured
+// with the rest).  offs and the per-plane parameters are host arrays.
+// `source` tells the graph cache which kind of input `prepare` reads.
+template <class Prep>
+void flux_run(gimic_ctx *c, int64_t npts, int nplanes, const int64_t *offs, const double *normals,
+              const double *bfields, const double *centers, const double *radii, int spin,
+              int precision, double *out, double *out_mod, double *out_acid, double *jt,
+              std::vector<uintptr_t> source, Prep &&prepare) {
+    hipStream_t st = c->stream;
+    std::vector<double> params(10 * (size_t)nplanes);
+    for (int k = 0; k < nplanes; k++) {
+        for (int d = 0; d < 3; d++) {
+            params[3 * k + d] = normals[3 * k + d];
+            params[3 * nplanes + 3 * k + d] = bfields[3 * k + d];
+            params[6 * nplanes + 3 * k + d] = centers[3 * k + d];
+        }
+        params[9 * nplanes + k] = radii[k];
+    }
+    const size_t np = (size_t)std::max<int64_t>(npts, 1);
+    double *d_r = c->pts_tmp.as<double>(), *d_w = d_r + 3 * np;
+    hip_check(hipMemcpyAsync(c->pparams.p, params.data(), params.size() * 8,
+                             hipMemcpyHostToDevice, st), "copy params");
+    hip_check(hipMemcpyAsync(c->offs.p, offs, (size_t)(nplanes + 1) * 8, hipMemcpyHostToDevice, st),
+              "copy offs");
+    FluxDev fd;
+    fd.w = d_w;
+    fd.plane = c->planes.as<int>();
+    const double *pp = c->pparams.as<double>();
+    fd.pnorm = pp;
+    fd.pbfield = pp + 3 * nplanes;
+    fd.pcenter = pp + 6 * nplanes;
+    fd.pradius = pp + 9 * nplanes;
+    fd.contrib = c->contrib.as<double>();
+    fd.contrib_mod = out_mod ? c->contrib.as<double>() + np : nullptr;
+    fd.contrib_acid = out_acid ? c->contrib.as<double>() + 2 * np : nullptr;
+    auto reduce = [&] {
+        for (int k = 0; k < 3; k++) {
+            if ((k == 1 && !out_mod) || (k == 2 && !out_acid)) continue;
+            hip_check(launch_plane_reduce(c->contrib.as<double>() + k * np,
+                                          (const long long *)c->offs.p, nplanes,
+                                          c->outs.as<double>() + 3 * k * nplanes, st),
+                      "plane_reduce");
+        }
+    };
+    if (!c->graphs) {
+        prepare(st);
+        evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd);
+        reduce();
+    } else {
+        // the inputs' preparation, all planes, every chunk and the per-plane
+        // reductions as one graph
+        evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd,
+                 EvalMode::Reserve);
+        auto &M = get_mats(c, {spin}, precision);
+        std::vector<uintptr_t> key = {
+            (uintptr_t)npts, (uintptr_t)nplanes, (uintptr_t)precision, (uintptr_t)spin,
+            (uintptr_t)(out_mod != nullptr), (uintptr_t)(out_acid != nullptr),
+            (uintptr_t)c->giao, (uintptr_t)c->diamag, (uintptr_t)c->paramag,
+            (uintptr_t)c->mixed_variant, (uintptr_t)st, (uintptr_t)c->pts_tmp.p,
+            (uintptr_t)c->jt_tmp.p, (uintptr_t)c->contrib.p, (uintptr_t)c->planes.p,
+            (uintptr_t)c->pparams.p, (uintptr_t)c->offs.p, (uintptr_t)c->outs.p,
+            (uintptr_t)c->phi64.p, (uintptr_t)c->grad64.p, (uintptr_t)c->phi32.p,
+            (uintptr_t)c->grad32.p, (uintptr_t)c->part.p, (uintptr_t)M.b64.p,
+            (uintptr_t)M.b32.p, (uintptr_t)c->phis.p, (uintptr_t)M.bs3.p,
+            (uintptr_t)c->mixed_split, (uintptr_t)c->spts.p, (uintptr_t)c->perm.p,
+            (uintptr_t)c->bpart.p, (uintptr_t)c->sortscr.p, (uintptr_t)c->bcount.p,
+            (uintptr_t)c->blist.p, (uintptr_t)c->toff.p};
+        key.insert(key.end(), source.begin(), source.end());
+        if (!c->graph_exec || key != c->graph_key) {
+            if (c->graph_exec) {
+                hip_check(hipGraphExecDestroy(c->graph_exec), "hipGraphExecDestroy");
+                c->graph_exec = nullptr;
+            }
+            hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal),
+                      "hipStreamBeginCapture");
+            hipGraph_t g = nullptr;
+            try {
+                prepare(st);
+                evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd,
+                         EvalMode::Capture);
+                reduce();
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(st, &g), "hipStreamEndCapture");
+            hipError_t e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hip_check(e, "hipGraphInstantiate");
+            c->graph_key = key;
+            c->graph_builds++;
+        }
+        hip_check(hipGraphLaunch(c->graph_exec, st), "hipGraphLaunch");
+        c->graph_launches++;
+    }
+    std::vector<double> res(9 * (size_t)nplanes);
+    hip_check(hipMemcpyAsync(res.data(), c->outs.p, res.size() * 8, hipMemcpyDeviceToHost, st),
+              "copy out");
+    if (jt && npts > 0)
+        hip_check(hipMemcpyAsync(jt, c->jt_tmp.p, (size_t)npts * 72, hipMemcpyDeviceToHost, st),
+                  "copy jt");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    std::copy(res.begin(), res.begin() + 3 * nplanes, out);
+    if (out_mod) std::copy(res.begin() + 3 * nplanes, res.begin() + 6 * nplanes, out_mod);
+    if (out_acid)
+        for (int k = 0; k < nplanes; k++) out_acid[k] = res[6 * (size_t)nplanes + 3 * k];
+}
+
+}  // namespace
+
+extern "C" {
+
 int gimic_flux_planes(gimic_ctx *c, int nplanes, const int64_t *offs, const double *r,
                       const double *w, const double *normals, const double *bfields,
                       const double *centers, const double *radii, int spin, int precision,
@@ -924,114 +1060,98 @@ int gimic_flux_planes(gimic_ctx *c, int nplanes, const int64_t *offs, const doub
             if (offs[k + 1] < offs[k]) throw Error(GIMIC_EINVAL, "plane_offs must be non-decreasing");
         const int64_t npts = offs[nplanes];
         hipStream_t st = c->stream;
-        std::vector<int> plane(npts);
-        for (int k = 0; k < nplanes; k++)
-            for (int64_t i = offs[k]; i < offs[k + 1]; i++) plane[i] = k;
-        std::vector<double> params(10 * (size_t)nplanes);
-        for (int k = 0; k < nplanes; k++) {
-            for (int d = 0; d < 3; d++) {
-                params[3 * k + d] = normals[3 * k + d];
-                params[3 * nplanes + 3 * k + d] = bfields[3 * k + d];
-                params[6 * nplanes + 3 * k + d] = centers[3 * k + d];
-            }
-            params[9 * nplanes + k] = radii[k];
-        }
+        flux_buffers(c, npts, nplanes);
         const size_t np = (size_t)std::max<int64_t>(npts, 1);
-        c->pts_tmp.alloc(np * 24 + np * 8, st, false);  // points then weights
-        c->jt_tmp.alloc(np * 72, st, false);
-        c->contrib.alloc(np * 24, st, true);  // current, signed modulus, ACID
-        c->planes.alloc(np * 4, st, false);
-        c->pparams.alloc(params.size() * 8, st, false);
-        c->offs.alloc((size_t)(nplanes + 1) * 8, st, false);
-        c->outs.alloc((size_t)nplanes * 72, st, false);
         double *d_r = c->pts_tmp.as<double>(), *d_w = d_r + 3 * np;
-        hip_check(hipMemcpyAsync(d_r, r, (size_t)npts * 24, hipMemcpyHostToDevice, st), "copy r");
-        hip_check(hipMemcpyAsync(d_w, w, (size_t)npts * 8, hipMemcpyHostToDevice, st), "copy w");
-        hip_check(hipMemcpyAsync(c->planes.p, plane.data(), (size_t)npts * 4,
-                                 hipMemcpyHostToDevice, st), "copy planes");
-        hip_check(hipMemcpyAsync(c->pparams.p, params.data(), params.size() * 8,
-                                 hipMemcpyHostToDevice, st), "copy params");
-        hip_check(hipMemcpyAsync(c->offs.p, offs, (size_t)(nplanes + 1) * 8,
-                                 hipMemcpyHostToDevice, st), "copy offs");
-        FluxDev fd;
-        fd.w = d_w;
-        fd.plane = c->planes.as<int>();
-        const double *pp = c->pparams.as<double>();
-        fd.pnorm = pp;
-        fd.pbfield = pp + 3 * nplanes;
-        fd.pcenter = pp + 6 * nplanes;
-        fd.pradius = pp + 9 * nplanes;
-        fd.contrib = c->contrib.as<double>();
-        fd.contrib_mod = out_mod ? c->contrib.as<double>() + np : nullptr;
-        fd.contrib_acid = out_acid ? c->contrib.as<double>() + 2 * np : nullptr;
-        auto reduce = [&] {
-            for (int k = 0; k < 3; k++) {
-                if ((k == 1 && !out_mod) || (k == 2 && !out_acid)) continue;
-                hip_check(launch_plane_reduce(c->contrib.as<double>() + k * np,
-                                              (const long long *)c->offs.p, nplanes,
-                                              c->outs.as<double>() + 3 * k * nplanes, st),
-                          "plane_reduce");
-            }
-        };
-        if (!c->graphs) {
-            evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd);
-            reduce();
-        } else {
-            // all planes, every chunk and the per-plane reductions as one graph
-            evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd,
-                     EvalMode::Reserve);
-            auto &M = get_mats(c, {spin}, precision);
-            std::vector<uintptr_t> key = {
-                (uintptr_t)npts, (uintptr_t)nplanes, (uintptr_t)precision, (uintptr_t)spin,
-                (uintptr_t)(out_mod != nullptr), (uintptr_t)(out_acid != nullptr),
-                (uintptr_t)c->giao, (uintptr_t)c->diamag, (uintptr_t)c->paramag,
-                (uintptr_t)c->mixed_variant, (uintptr_t)st, (uintptr_t)c->pts_tmp.p,
-                (uintptr_t)c->jt_tmp.p, (uintptr_t)c->contrib.p, (uintptr_t)c->planes.p,
-                (uintptr_t)c->pparams.p, (uintptr_t)c->offs.p, (uintptr_t)c->outs.p,
-                (uintptr_t)c->phi64.p, (uintptr_t)c->grad64.p, (uintptr_t)c->phi32.p,
-                (uintptr_t)c->grad32.p, (uintptr_t)c->part.p, (uintptr_t)M.b64.p,
-                (uintptr_t)M.b32.p, (uintptr_t)c->phis.p, (uintptr_t)M.bs3.p,
-                (uintptr_t)c->mixed_split, (uintptr_t)c->spts.p, (uintptr_t)c->perm.p,
-                (uintptr_t)c->bpart.p, (uintptr_t)c->sortscr.p, (uintptr_t)c->bcount.p,
-                (uintptr_t)c->blist.p, (uintptr_t)c->toff.p};
-            if (!c->graph_exec || key != c->graph_key) {
-                if (c->graph_exec) {
-                    hip_check(hipGraphExecDestroy(c->graph_exec), "hipGraphExecDestroy");
-                    c->graph_exec = nullptr;
-                }
-                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal),
-                          "hipStreamBeginCapture");
-                hipGraph_t g = nullptr;
-                try {
-                    evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd,
-                             EvalMode::Capture);
-                    reduce();
-                } catch (...) {
-                    (void)hipStreamEndCapture(st, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    throw;
-                }
-                hip_check(hipStreamEndCapture(st, &g), "hipStreamEndCapture");
-                hipError_t e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                hip_check(e, "hipGraphInstantiate");
-                c->graph_key = key;
-                c->graph_builds++;
-            }
-            hip_check(hipGraphLaunch(c->graph_exec, st), "hipGraphLaunch");
-            c->graph_launches++;
+        if (npts > 0) {
+            hip_check(hipMemcpyAsync(d_r, r, (size_t)npts * 24, hipMemcpyHostToDevice, st), "copy r");
+            hip_check(hipMemcpyAsync(d_w, w, (size_t)npts * 8, hipMemcpyHostToDevice, st), "copy w");
         }
-        std::vector<double> res(9 * (size_t)nplanes);
-        hip_check(hipMemcpyAsync(res.data(), c->outs.p, res.size() * 8, hipMemcpyDeviceToHost, st),
-                  "copy out");
-        if (jt)
-            hip_check(hipMemcpyAsync(jt, c->jt_tmp.p, (size_t)npts * 72, hipMemcpyDeviceToHost,
-                                     st), "copy jt");
-        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-        std::copy(res.begin(), res.begin() + 3 * nplanes, out);
-        if (out_mod) std::copy(res.begin() + 3 * nplanes, res.begin() + 6 * nplanes, out_mod);
-        if (out_acid)
-            for (int k = 0; k < nplanes; k++) out_acid[k] = res[6 * (size_t)nplanes + 3 * k];
+        flux_run(c, npts, nplanes, offs, normals, bfields, centers, radii, spin, precision, out,
+                 out_mod, out_acid, jt, {(uintptr_t)0}, [&](hipStream_t s) {
+                     hip_check(launch_plane_index((const long long *)c->offs.p, nplanes, npts,
+                                                  c->planes.as<int>(), s),
+                               "plane_index");
+                 });
+    });
+}
+
+int gimic_flux_plane_grids(gimic_ctx *c, int nplanes, const int *n1, const int *n2,
+                           const double *origins, const double *axes, const double *p3w3,
+                           const double *nodes1, const double *wts1, const double *nodes2,
+                           const double *wts2, const double *normals, const double *bfields,
+                           const double *centers, const double *radii, int spin, int precision,
+                           double *out, double *out_mod, double *out_acid) {
+    return guarded([&] {
+        if (!c || nplanes < 1 || !n1 || !n2 || !origins || !axes || !p3w3 || !nodes1 || !wts1 ||
+            !nodes2 || !wts2 || !normals || !bfields || !centers || !radii || !out)
+            throw Error(GIMIC_EINVAL, "null argument");
+        // host side: O(nplanes) offsets; the device generates the points
+        std::vector<int64_t> poff(nplanes + 1, 0), off1(nplanes, 0), off2(nplanes, 0);
+        int64_t s1 = 0, s2 = 0;
+        for (int k = 0; k < nplanes; k++) {
+            if (n1[k] < 1 || n2[k] < 1) throw Error(GIMIC_EINVAL, "every plane needs n1, n2 >= 1");
+            off1[k] = s1;
+            off2[k] = s2;
+            s1 += n1[k];
+            s2 += n2[k];
+            poff[k + 1] = poff[k] + (int64_t)n1[k] * n2[k];
+        }
+        const int64_t npts = poff[nplanes];
+        hipStream_t st = c->stream;
+        flux_buffers(c, npts, nplanes);
+        // one spec buffer: n1 | off1 off2 poff | origin axes p3w3 nodes1 wts1 nodes2 wts2
+        const size_t ni = (size_t)nplanes, nl = 3 * ni + 1,
+                     nd = 3 * ni + 9 * ni + 2 * ni + 2 * (size_t)s1 + 2 * (size_t)s2;
+        std::vector<unsigned char> spec(round_up(ni * 4, 8) + nl * 8 + nd * 8);
+        unsigned char *q = spec.data();
+        std::memcpy(q, n1, ni * 4);
+        int64_t *lq = (int64_t *)(q + round_up(ni * 4, 8));
+        std::memcpy(lq, off1.data(), ni * 8);
+        std::memcpy(lq + ni, off2.data(), ni * 8);
+        std::memcpy(lq + 2 * ni, poff.data(), (ni + 1) * 8);
+        double *dq = (double *)(lq + nl);
+        size_t o = 0;
+        auto put = [&](const double *src, size_t n) {
+            std::memcpy(dq + o, src, n * 8);
+            o += n;
+        };
+        put(origins, 3 * ni);
+        put(axes, 9 * ni);
+        put(p3w3, 2 * ni);
+        put(nodes1, (size_t)s1);
+        put(wts1, (size_t)s1);
+        put(nodes2, (size_t)s2);
+        put(wts2, (size_t)s2);
+        c->gspec.alloc(spec.size(), st, false);
+        hip_check(hipMemcpyAsync(c->gspec.p, spec.data(), spec.size(), hipMemcpyHostToDevice, st),
+                  "copy grid spec");
+        const unsigned char *dbase = (const unsigned char *)c->gspec.p;
+        const long long *dl = (const long long *)(dbase + round_up(ni * 4, 8));
+        const double *dd = (const double *)(dl + nl);
+        PlaneGridArgs ga{};
+        ga.nplanes = nplanes;
+        ga.npts = npts;
+        ga.n1 = (const int *)dbase;
+        ga.off1 = dl;
+        ga.off2 = dl + ni;
+        ga.poff = dl + 2 * ni;
+        ga.origin = dd;
+        ga.axes = dd + 3 * ni;
+        ga.p3w3 = dd + 12 * ni;
+        ga.nodes1 = dd + 14 * ni;
+        ga.wts1 = ga.nodes1 + s1;
+        ga.nodes2 = ga.wts1 + s1;
+        ga.wts2 = ga.nodes2 + s2;
+        const size_t np = (size_t)std::max<int64_t>(npts, 1);
+        double *d_r = c->pts_tmp.as<double>(), *d_w = d_r + 3 * np;
+        flux_run(c, npts, nplanes, poff.data(), normals, bfields, centers, radii, spin, precision,
+                 out, out_mod, out_acid, nullptr,
+                 {(uintptr_t)1, (uintptr_t)c->gspec.p, (uintptr_t)s1, (uintptr_t)s2},
+                 [&](hipStream_t s) {
+                     hip_check(launch_plane_grid(ga, d_r, d_w, c->planes.as<int>(), s),
+                               "plane_grid");
+                 });
     });
 }
 

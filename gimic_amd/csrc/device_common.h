@@ -174,6 +174,22 @@ hipError_t launch_properties(const double *jt, const double *r, const double *w,
 // p[0..n) = 0 (a kernel, so it may be captured into a graph)
 hipError_t launch_fill_zero(double *p, size_t n, hipStream_t s);
 
+// Plane-flux inputs on the device (planes.hip): plane index of each point
+// from the plane offsets, and bond-plane product grids from their
+// descriptors (per plane: n1 x n2 points, origin, axes v1 v2 v3, p3 and w3;
+// 1-D nodes and weights at off1 / off2; point offsets poff[nplanes + 1])
+struct PlaneGridArgs {
+    int nplanes;
+    long long npts;
+    const int *n1;
+    const long long *off1, *off2, *poff;
+    const double *origin, *axes, *p3w3, *nodes1, *wts1, *nodes2, *wts2;
+};
+hipError_t launch_plane_index(const long long *offs, int nplanes, long long npts, int *plane,
+                              hipStream_t s);
+hipError_t launch_plane_grid(const PlaneGridArgs &a, double *r, double *w, int *plane,
+                             hipStream_t s);
+
 // deterministic per-plane sums of contributions: out[3*plane] = net, pos, neg
 hipError_t launch_plane_reduce(const double *contrib, const long long *offs, int nplanes,
                                double *out, hipStream_t s);

@@ -111,6 +111,27 @@ module gimic_hip
             type(c_ptr), value :: out_mod, out_acid, jt
             integer(c_int) :: rc
         end function
+        ! the same sums over product grids generated on the device: origin,
+        ! axis vectors (basv columns, grid.f90) and 1-D rules per plane
+        function gimic_flux_plane_grids(ctx, nplanes, n1, n2, origins, axes, p3w3, &
+                nodes1, wts1, nodes2, wts2, normals, bfields, centers, radii, spin, &
+                precision, out, out_mod, out_acid) &
+                bind(c, name='gimic_flux_plane_grids') result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nplanes
+            integer(c_int), dimension(*), intent(in) :: n1, n2
+            real(c_double), dimension(3, *), intent(in) :: origins
+            real(c_double), dimension(9, *), intent(in) :: axes
+            real(c_double), dimension(2, *), intent(in) :: p3w3
+            real(c_double), dimension(*), intent(in) :: nodes1, wts1, nodes2, wts2
+            real(c_double), dimension(3, *), intent(in) :: normals, bfields, centers
+            real(c_double), dimension(*), intent(in) :: radii
+            integer(c_int), value :: spin, precision
+            real(c_double), dimension(3, *), intent(out) :: out
+            type(c_ptr), value :: out_mod, out_acid
+            integer(c_int) :: rc
+        end function
         ! get_property (jfield.f90:584-929): shielding sums per nucleus and atom
         ! block, magnetizability sums per atom block, 5 values each
         function gimic_ctx_properties(ctx, npts, r, w, npart, part_offs, nnuc, nuclei, &

@@ -201,6 +201,26 @@ int gimic_flux_planes(gimic_ctx *ctx, int nplanes, const int64_t *plane_offs, co
                       const double *centers, const double *radii, int spin, int precision,
                       double *out, double *out_mod, double *out_acid, double *jt);
 
+/* Plane flux over product-quadrature planes generated on the device (the
+ * bond and base grids of grid.f90, integral.f90:50-186 with the points of
+ * gridpoint, grid.f90:478-495; no separate entry point in the reference).
+ * Plane k has n1[k] x n2[k] points, i fastest:
+ *   r = ((o + p1_i v1) + p2_j v2) + p3 v3,   w = (w3 w2_j) w1_i
+ * with o = origins[3k..], v1, v2, v3 = axes[9k..], axes[9k+3..], axes[9k+6..],
+ * (p3, w3) = p3w3[2k], p3w3[2k+1]; the 1-D rules of all planes are
+ * concatenated in plane order (nodes1/wts1: sum n1 values, nodes2/wts2: sum
+ * n2).  Each operation is rounded as written (no fused multiply-add), so the
+ * points and the sums equal those of gimic_flux_planes on host-built arrays
+ * bit for bit.  normals ... out_acid as for gimic_flux_planes.  Only
+ * O(sum n1 + n2) numbers cross to the device; in graph mode the generator
+ * runs inside the captured graph. */
+int gimic_flux_plane_grids(gimic_ctx *ctx, int nplanes, const int *n1, const int *n2,
+                           const double *origins, const double *axes, const double *p3w3,
+                           const double *nodes1, const double *wts1, const double *nodes2,
+                           const double *wts2, const double *normals, const double *bfields,
+                           const double *centers, const double *radii, int spin, int precision,
+                           double *out, double *out_mod, double *out_acid);
+
 /* Magnetic properties on an external quadrature grid (jfield.f90:584-929,
  * Essential.prop with numgrid files).  J at npts points r[3*i..] with
  * weights w[i]; the points fall into npart consecutive partitions (the

@@ -106,3 +106,29 @@ def test_inp_parser_reads_fixture_keywords():
     assert kw["Grid"]["_arg"] == "bond" and kw["Grid"]["gauss_order"] == 9
     assert kw["Grid"]["bond"] == [2, 1] and kw["Grid"]["width"] == [-1.25614, 6.0]
     assert kw["Advanced"]["screening_thrs"] == 1e-8 and kw["Advanced"]["spherical"] is False
+
+
+def test_plane_grid_generator_rounding_matches_grid_points():
+    """the device generator's arithmetic, ((o + p1 v1) + p2 v2) + p3 v3 and
+    (w3 w2) w1 each rounded once per operation, restated in numpy, equals
+    Grid.points() and Grid.weights() bit for bit on an uneven bond plane"""
+    from gimic_amd import grid as gridmod
+    coords = np.array([[0.0, 0.0, 0.0], [1.39, 0.21, -0.05], [2.1, 1.4, 0.3]])
+    kw = {"_arg": "bond", "type": "lobatto", "gauss_order": 7, "bond": [1, 2],
+          "fixcoord": [0.3, -0.7, 2.0], "distance": 0.37, "height": [-4.0, 4.5],
+          "width": [-2.2, 9.8], "grid_points": [33, 47, 0]}
+    g = gridmod.new_grid(kw, coords)
+    p1, p2, p3 = g.pts
+    w1, w2, w3 = g.wgt
+    n1, n2 = len(p1), len(p2)
+    idx = np.arange(n1 * n2 * len(p3))
+    i, j, k = idx % n1, (idx // n1) % n2, idx // (n1 * n2)
+    v = g.basv
+    r = np.empty((len(idx), 3))
+    for d in range(3):
+        x = g.origin[d] + p1[i] * v[d, 0]
+        x = x + p2[j] * v[d, 1]
+        r[:, d] = x + p3[k] * v[d, 2]
+    w = (w3[k] * w2[j]) * w1[i]
+    assert np.array_equal(r, g.points())
+    assert np.array_equal(w, g.weights())

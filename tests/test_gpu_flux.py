@@ -150,3 +150,63 @@ def test_graph_launch_equals_eager(precision):
         assert np.array_equal(g, e)
     assert not np.array_equal(got2[0], eager[0])
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_plane_grids_equal_host_arrays(precision):
+    """points and weights generated on the device from the 1-D rules give the
+    same sums, bit for bit, as the host-built arrays, eager and in a graph
+    (replays, then new geometry of the same sizes); a grid with three layers
+    in p3 counts as three planes whose sums add up to the layered grid's"""
+    from gimic_amd import flux, grid as gridmod, synth
+    s = synth.ring12()
+    ctx = s.context()
+    coords = np.asarray(s.coords)
+
+    def grids_at(shift):
+        gs, bs = [], []
+        for k, dist in enumerate(np.linspace(-2.0, 3.0, 7)):
+            kw = {"_arg": "bond", "type": "lobatto", "gauss_order": 7, "bond": [1, 2],
+                  "fixcoord": [0.0, 0.0, 0.0], "distance": float(dist + shift),
+                  "height": [-4.0, 4.0], "width": [-2.2, 9.8],
+                  "grid_points": [14 + 7 * k, 21 + 7 * ((3 * k) % 5), 0]}
+            g = gridmod.new_grid(kw, coords)
+            gs.append(g)
+            bs.append(gridmod.get_magnet(g, "X"))
+        return gs, bs
+
+    grids, fields = grids_at(0.0)
+    host = [flux.plane_of(g, b) for g, b in zip(grids, fields)]
+    print("case device plane grids eager", precision, flush=True)
+    ref = ctx.flux_planes(host, precision=precision, modulus=True, acid=True)
+    got = ctx.flux_plane_grids(grids, fields, precision=precision, modulus=True, acid=True)
+    for g, e in zip(got, ref):
+        assert np.array_equal(g, e), (g, e)
+    print("case device plane grids graph", precision, flush=True)
+    ctx.set_graph(True)
+    rng = np.random.default_rng(11)
+    b0, _ = ctx.graph_stats()
+    for rep in range(3):
+        got = ctx.flux_plane_grids(grids, fields, precision=precision, modulus=True, acid=True)
+        for g, e in zip(got, ref):
+            assert np.array_equal(g, e), (rep, g, e)
+        ctx.jtensor(rng.normal(0, 2, (300, 3)), precision=precision)
+    grids2, fields2 = grids_at(0.41)
+    got2 = ctx.flux_plane_grids(grids2, fields2, precision=precision, modulus=True, acid=True)
+    assert ctx.graph_stats()[0] == b0 + 1  # one capture for the three replays and the move
+    ctx.set_graph(False)
+    ref2 = ctx.flux_planes([flux.plane_of(g, b) for g, b in zip(grids2, fields2)],
+                           precision=precision, modulus=True, acid=True)
+    for g, e in zip(got2, ref2):
+        assert np.array_equal(g, e)
+    assert not np.array_equal(got2[0], ref[0])
+    # three layers along v3
+    g3 = grids[2]
+    g3.pts[2] = np.array([-0.2, 0.0, 0.25])
+    g3.wgt[2] = np.array([0.3, 0.5, 0.2])
+    print("case device plane grids layered", precision, flush=True)
+    lay = ctx.flux_plane_grids([g3], [fields[2]], precision=precision, modulus=True)
+    one = ctx.flux_planes([flux.plane_of(g3, fields[2])], precision=precision, modulus=True)
+    for g, e in zip(lay, one):
+        assert np.allclose(g, e, rtol=1e-12, atol=1e-16), (g, e)
+    ctx.close()
