@@ -89,6 +89,117 @@ struct DevBuf {
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
+// Expected cost of a function order: the mean over sample points of
+// (number of live 32-function blocks)^2, the live (K-block, N-block) pairs
+// of a row tile at the point.  live[p * nsh + s]: shell s reaches sample p.
+template <class NOut>
+double order_cost(const Basis &b, const std::vector<int> &order, const std::vector<char> &live,
+                  int npt, NOut nout_of) {
+    const int nsh = (int)b.shells.size();
+    std::vector<int> fpos(nsh), fcnt(nsh);
+    int pos = 0;
+    for (int si : order) {
+        fpos[si] = pos;
+        fcnt[si] = nout_of(b.shells[si]);
+        pos += fcnt[si];
+    }
+    const int nblk = (pos + 31) / 32;
+    std::vector<int> stamp(nblk, -1);
+    double sum = 0.0;
+    for (int p = 0; p < npt; p++) {
+        const char *lv = live.data() + (size_t)p * nsh;
+        int n = 0;
+        for (int s = 0; s < nsh; s++) {
+            if (!lv[s]) continue;
+            for (int k = fpos[s] >> 5; k <= (fpos[s] + fcnt[s] - 1) >> 5; k++)
+                if (stamp[k] != p) {
+                    stamp[k] = p;
+                    n++;
+                }
+        }
+        sum += (double)n * n;
+    }
+    return sum / npt;
+}
+
+// Candidate orders: the plain order by decreasing radius, and geometric
+// radius buckets of ratio q and phase ph (bucket floor(log r / log q + ph),
+// largest first; within a bucket by atom rank, then decreasing radius).
+// Scored by order_cost on deterministic sample points spread over the
+// molecule's bounding box widened by 1 bohr, with every radius widened by
+// 0.5 bohr for the extent of a row tile; the first best candidate wins.
+template <class NOut>
+std::vector<int> choose_function_order(const Basis &b, const std::vector<int> &arank,
+                                       NOut nout_of) {
+    const int nsh = (int)b.shells.size(), nat = b.natoms();
+    std::vector<int> base(nsh);
+    for (int i = 0; i < nsh; i++) base[i] = i;
+    std::stable_sort(base.begin(), base.end(), [&](int x, int y) {
+        const Shell &p = b.shells[x], &q = b.shells[y];
+        if (p.radius != q.radius) return p.radius > q.radius;
+        return arank[p.atom] < arank[q.atom];
+    });
+    if (nsh < 2 || nat < 2) return base;
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+        lo[d] = hi[d] = b.coords[d];
+        for (int a = 1; a < nat; a++) {
+            lo[d] = std::min(lo[d], b.coords[3 * a + d]);
+            hi[d] = std::max(hi[d], b.coords[3 * a + d]);
+        }
+        lo[d] -= 1.0;
+        hi[d] += 1.0;
+    }
+    // about 4096 samples, fewer for very large bases (live holds npt * nsh bytes)
+    const int npt = (int)std::max<long long>(256, std::min<long long>(4096, 24000000LL / nsh));
+    std::vector<char> live((size_t)npt * nsh);
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    auto unif = [&st] {  // xorshift64*, fixed seed: the choice repeats run to run
+        st ^= st >> 12;
+        st ^= st << 25;
+        st ^= st >> 27;
+        return (double)((st * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int p = 0; p < npt; p++) {
+        double x[3];
+        for (int d = 0; d < 3; d++) x[d] = lo[d] + unif() * (hi[d] - lo[d]);
+        for (int s = 0; s < nsh; s++) {
+            const Shell &sh = b.shells[s];
+            double r2 = 0.0;
+            for (int d = 0; d < 3; d++) {
+                const double e = x[d] - b.coords[3 * sh.atom + d];
+                r2 += e * e;
+            }
+            const double r = sh.radius + 0.5;
+            live[(size_t)p * nsh + s] = r2 <= r * r;
+        }
+    }
+    std::vector<int> best = base;
+    double best_cost = order_cost(b, base, live, npt, nout_of);
+    const double qs[] = {1.25, 1.4, 1.6, 1.8, 2.0, 2.5, 3.2, 4.5};
+    const double phs[] = {0.0, 0.25, 0.5, 0.75};
+    std::vector<double> key(nsh);
+    for (double q : qs)
+        for (double ph : phs) {
+            for (int i = 0; i < nsh; i++)
+                key[i] = std::floor(std::log(b.shells[i].radius) / std::log(q) + ph);
+            std::vector<int> o(nsh);
+            for (int i = 0; i < nsh; i++) o[i] = i;
+            std::stable_sort(o.begin(), o.end(), [&](int x, int y) {
+                if (key[x] != key[y]) return key[x] > key[y];
+                const Shell &p = b.shells[x], &r = b.shells[y];
+                if (arank[p.atom] != arank[r.atom]) return arank[p.atom] < arank[r.atom];
+                return p.radius > r.radius;
+            });
+            const double c = order_cost(b, o, live, npt, nout_of);
+            if (c < best_cost) {
+                best_cost = c;
+                best = o;
+            }
+        }
+    return best;
+}
+
 }  // namespace
 
 struct gimic_ctx {
@@ -124,6 +235,12 @@ struct gimic_ctx {
     DevBuf prop_in, prop_jt, prop_offs, prop_nuc, prop_out;
     // plane-grid descriptors (gimic_flux_plane_grids)
     DevBuf gspec;
+    // mixed precision: points within fp64_radius bohr of a nucleus are
+    // evaluated again in FP64 (0: off); selection index list, their points,
+    // selection scratch and count; points refined by the last mixed call
+    double fp64_radius = 0.4;
+    DevBuf near_idx, near_pts, near_scr, near_cnt;
+    long long refined_last = 0;
     // optional per-kernel timing with HIP events on the launch stream:
     // [0] basis evaluation, [1] contraction, [2] finalize
     bool timing = false;
@@ -177,12 +294,15 @@ void setup_device(gimic_ctx *c) {
     finalize_basis(b, c->screen_thrs);
     c->lda = (int)round_up((size_t)b.nbf, kKPad);
     c->ncol_pad = (int)round_up((size_t)b.nbf, 32);
-    // Internal function order: shells by decreasing screening radius, then
-    // by atom along a nearest-neighbour path through the molecule, so that a
-    // 32-function block holds functions of like range centred close
-    // together and screening zeroes whole blocks over a compact tile of
-    // points.  Within a shell the order is the caller's.  J is a sum over
-    // function pairs, so only the summation order changes.
+    // Internal function order: shells grouped into classes of like screening
+    // radius (geometric buckets, largest first), within a class by atom
+    // along a nearest-neighbour path through the molecule, then by
+    // decreasing radius; so a 32-function block holds functions of like
+    // range centred close together and screening zeroes whole blocks over
+    // a compact tile of points.  The bucket ratio and phase are chosen per
+    // basis by the expected count of live block pairs (order_cost).  Within
+    // a shell the order is the caller's.  J is a sum over function pairs,
+    // so only the summation order changes.
     const int nsh = (int)b.shells.size(), nat = b.natoms();
     std::vector<int> arank(std::max(nat, 1), 0);
     {
@@ -209,14 +329,8 @@ void setup_device(gimic_ctx *c) {
             cur = best;
         }
     }
-    std::vector<int> order(nsh);
-    for (int i = 0; i < nsh; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        const Shell &p = b.shells[x], &q = b.shells[y];
-        if (p.radius != q.radius) return p.radius > q.radius;
-        return arank[p.atom] < arank[q.atom];
-    });
     auto nout_of = [&](const Shell &s) { return b.spherical ? 2 * s.l + 1 : ncart(s.l); };
+    const std::vector<int> order = choose_function_order(b, arank, nout_of);
     std::vector<int> fnew(nsh, 0);
     c->fun_orig.assign(b.nbf, 0);
     {
@@ -444,10 +558,18 @@ enum class EvalMode {
     Capture,  // kernel launches only (no allocation, memset, timing or sync)
 };
 
+void refine_near_nuclei(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
+                        const std::vector<int> &spins, hipStream_t st, const FluxDev *flux);
+
 // Evaluate J for npts device points into d_jt (nspin blocks of 9*npts).
+// With out_idx (a subset run), point i's tensor and flux contribution go to
+// index out_idx[i] of output arrays sized for out_npts points.  A mixed
+// evaluation in eager mode ends with the FP64 pass over the points next to
+// a nucleus (refine_near_nuclei); a captured one leaves it to the caller.
 void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
               const std::vector<int> &spins, int prec, hipStream_t st, const FluxDev *flux,
-              EvalMode mode = EvalMode::Eager) {
+              EvalMode mode = EvalMode::Eager, const long long *out_idx = nullptr,
+              int64_t out_npts = 0) {
     if (npts <= 0) return;
     if (prec != GIMIC_PREC_FP64 && prec != GIMIC_PREC_MIXED)
         throw Error(GIMIC_EINVAL, "unknown precision");
@@ -720,21 +842,56 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         fa.giao = c->giao;
         fa.diamag = c->diamag;
         fa.paramag = c->paramag;
-        fa.jt = d_jt + 9 * start;
-        fa.jt_stride = 9 * (long long)npts;
+        // outputs: this chunk's slice, or (subset run) the whole arrays
+        // addressed through out_idx
+        const size_t o0 = out_idx ? 0 : start;
+        fa.pidx = out_idx ? out_idx + start : nullptr;
+        fa.jt = d_jt + 9 * o0;
+        fa.jt_stride = 9 * (long long)(out_idx ? out_npts : npts);
         if (flux) {
-            fa.w = flux->w + start;
-            fa.plane = flux->plane ? flux->plane + start : nullptr;
+            fa.w = flux->w + o0;
+            fa.plane = flux->plane ? flux->plane + o0 : nullptr;
             fa.pnorm = flux->pnorm;
             fa.pbfield = flux->pbfield;
             fa.pcenter = flux->pcenter;
             fa.pradius = flux->pradius;
-            fa.contrib = flux->contrib + start;
-            fa.contrib_mod = flux->contrib_mod ? flux->contrib_mod + start : nullptr;
-            fa.contrib_acid = flux->contrib_acid ? flux->contrib_acid + start : nullptr;
+            fa.contrib = flux->contrib + o0;
+            fa.contrib_mod = flux->contrib_mod ? flux->contrib_mod + o0 : nullptr;
+            fa.contrib_acid = flux->contrib_acid ? flux->contrib_acid + o0 : nullptr;
         }
         timed(c, 2, st, [&] { hip_check(launch_finalize(fa, st), "finalize"); });
     }
+    if (prec == GIMIC_PREC_MIXED && mode == EvalMode::Eager && !out_idx)
+        refine_near_nuclei(c, npts, d_r, d_jt, spins, st, flux);
+}
+
+// The FP64 pass of a mixed evaluation: the points within fp64_radius of a
+// nucleus, where J is large and FP32 products lose the most relative
+// accuracy, are selected on the device (in index order), evaluated again in
+// FP64, and their tensors and flux contributions overwrite the mixed ones in
+// place.  Synchronises once, to size the subset.
+void refine_near_nuclei(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
+                        const std::vector<int> &spins, hipStream_t st, const FluxDev *flux) {
+    c->refined_last = 0;
+    if (!(c->fp64_radius > 0.0) || npts <= 0) return;
+    if (npts > 0x7fffffffLL) throw Error(GIMIC_EINVAL, "too many points for the FP64 pass");
+    c->near_idx.alloc((size_t)npts * 8, st, false);
+    c->near_scr.alloc(near_select_scratch(npts) * 4, st, false);
+    c->near_cnt.alloc(8, st, false);
+    hip_check(launch_near_select(d_r, npts, c->coords.as<double>(), c->basis.natoms(),
+                                 c->fp64_radius, c->near_idx.as<long long>(),
+                                 c->near_cnt.as<long long>(), c->near_scr.as<int>(), st),
+              "near_select");
+    long long k = 0;
+    hip_check(hipMemcpyAsync(&k, c->near_cnt.p, 8, hipMemcpyDeviceToHost, st), "copy count");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (k <= 0) return;
+    c->near_pts.alloc((size_t)k * 24, st, false);
+    hip_check(launch_gather_points(d_r, c->near_idx.as<long long>(), k, c->near_pts.as<double>(), st),
+              "gather_points");
+    evaluate(c, k, c->near_pts.as<double>(), d_jt, spins, GIMIC_PREC_FP64, st, flux,
+             EvalMode::Eager, c->near_idx.as<long long>(), npts);
+    c->refined_last = k;
 }
 
 // ---- drop-in global state (gimic_interface.f90:21-24) ----
@@ -978,6 +1135,9 @@ void flux_run(gimic_ctx *c, int64_t npts, int nplanes, const int64_t *offs, cons
                       "plane_reduce");
         }
     };
+    // the FP64 pass over points next to a nucleus runs eagerly after a
+    // captured mixed evaluation, and the reductions after it
+    const bool refine = precision == GIMIC_PREC_MIXED && c->fp64_radius > 0.0;
     if (!c->graphs) {
         prepare(st);
         evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd);
@@ -1000,7 +1160,7 @@ void flux_run(gimic_ctx *c, int64_t npts, int nplanes, const int64_t *offs, cons
             (uintptr_t)M.b32.p, (uintptr_t)c->phis.p, (uintptr_t)M.bs3.p,
             (uintptr_t)c->mixed_split, (uintptr_t)c->spts.p, (uintptr_t)c->perm.p,
             (uintptr_t)c->bpart.p, (uintptr_t)c->sortscr.p, (uintptr_t)c->bcount.p,
-            (uintptr_t)c->blist.p, (uintptr_t)c->toff.p};
+            (uintptr_t)c->blist.p, (uintptr_t)c->toff.p, (uintptr_t)refine};
         key.insert(key.end(), source.begin(), source.end());
         if (!c->graph_exec || key != c->graph_key) {
             if (c->graph_exec) {
@@ -1014,7 +1174,7 @@ void flux_run(gimic_ctx *c, int64_t npts, int nplanes, const int64_t *offs, cons
                 prepare(st);
                 evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd,
                          EvalMode::Capture);
-                reduce();
+                if (!refine) reduce();
             } catch (...) {
                 (void)hipStreamEndCapture(st, &g);
                 if (g) (void)hipGraphDestroy(g);
@@ -1029,6 +1189,10 @@ void flux_run(gimic_ctx *c, int64_t npts, int nplanes, const int64_t *offs, cons
         }
         hip_check(hipGraphLaunch(c->graph_exec, st), "hipGraphLaunch");
         c->graph_launches++;
+        if (refine) {
+            refine_near_nuclei(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, st, &fd);
+            reduce();
+        }
     }
     std::vector<double> res(9 * (size_t)nplanes);
     hip_check(hipMemcpyAsync(res.data(), c->outs.p, res.size() * 8, hipMemcpyDeviceToHost, st),
@@ -1175,6 +1339,23 @@ int gimic_ctx_set_mixed_kernel(gimic_ctx *c, int variant, int split) {
         c->mixed_variant = variant == 0 ? -1 : variant;
         c->mixed_split = variant >= 3 ? split : 0;
         for (auto &e : c->tune_ms) e = 0.f;
+    });
+}
+
+int gimic_ctx_set_fp64_radius(gimic_ctx *c, double radius) {
+    return guarded([&] {
+        if (!c) throw Error(GIMIC_EINVAL, "null context");
+        if (!(radius >= 0.0) || !std::isfinite(radius))
+            throw Error(GIMIC_EINVAL, "radius must be finite and >= 0");
+        c->fp64_radius = radius;
+    });
+}
+
+int gimic_ctx_fp64_refined(const gimic_ctx *c, double *radius, int64_t *npts) {
+    return guarded([&] {
+        if (!c) throw Error(GIMIC_EINVAL, "null context");
+        if (radius) *radius = c->fp64_radius;
+        if (npts) *npts = c->refined_last;
     });
 }
 

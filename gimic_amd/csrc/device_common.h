@@ -153,6 +153,7 @@ struct FinalizeArgs {
     int nsplit, rows, npts, nmatset;
     const double *pts;   // chunk points in row order
     const int *perm;     // caller index (within the chunk) of each row, or null
+    const long long *pidx;  // output index of each caller index (subset runs), or null
     int giao, diamag, paramag;
     double *jt;          // [nmatset][npts][9] (chunk offset applied)
     long long jt_stride; // elements between matrix sets in jt
@@ -173,6 +174,17 @@ hipError_t launch_properties(const double *jt, const double *r, const double *w,
                              double *out, hipStream_t s);
 // p[0..n) = 0 (a kernel, so it may be captured into a graph)
 hipError_t launch_fill_zero(double *p, size_t n, hipStream_t s);
+
+// Points within `radius` of any nucleus (refine.hip): idx[0..count) = their
+// indices in increasing order (count written to *count on the device);
+// scratch holds near_select_scratch(npts) ints.
+size_t near_select_scratch(long long npts);
+hipError_t launch_near_select(const double *r, long long npts, const double *coords, int natoms,
+                              double radius, long long *idx, long long *count, int *scratch,
+                              hipStream_t s);
+// out[3j..] = r[3 idx[j]..] for j < n
+hipError_t launch_gather_points(const double *r, const long long *idx, long long n, double *out,
+                                hipStream_t s);
 
 // Plane-flux inputs on the device (planes.hip): plane index of each point
 // from the plane offsets, and bond-plane product grids from their

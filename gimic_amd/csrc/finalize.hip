@@ -26,7 +26,8 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= a.npts) return;
     const double r[3] = {a.pts[3 * p], a.pts[3 * p + 1], a.pts[3 * p + 2]};
-    const int o = a.perm ? a.perm[p] : p;  // caller index of row p
+    long long o = a.perm ? a.perm[p] : p;  // caller index of row p
+    if (a.pidx) o = a.pidx[o];             // its place in the output arrays
     for (int set = 0; set < a.nmatset; set++) {
         double q[kNQ];
 #pragma unroll

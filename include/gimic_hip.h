@@ -148,6 +148,19 @@ int gimic_ctx_tuning_log(const gimic_ctx *ctx, int *n, int *codes, double *ms);
  * next mixed evaluation tunes again). */
 int gimic_ctx_set_mixed_kernel(gimic_ctx *ctx, int variant, int split);
 
+/* Mixed precision near nuclei (no counterpart in the reference): with
+ * GIMIC_PREC_MIXED, the points within `radius` bohr of any nucleus are
+ * evaluated again in FP64 after the mixed pass, and their tensors and flux
+ * contributions replace the mixed ones.  There the tight basis functions
+ * make J large and its terms cancel, and FP32 products lose most of their
+ * relative accuracy; a plane passing a nucleus at 0.1 bohr otherwise carries
+ * most of its flux error in a few points.  Default 0.4 bohr; 0 turns the
+ * pass off.  The selection synchronises the stream once per call.
+ * gimic_ctx_fp64_refined reports the radius and how many points the last
+ * mixed call evaluated in FP64 (either pointer may be NULL). */
+int gimic_ctx_set_fp64_radius(gimic_ctx *ctx, double radius);
+int gimic_ctx_fp64_refined(const gimic_ctx *ctx, double *radius, int64_t *npts);
+
 /* Screening block-list statistics (no counterpart in the reference).  While
  * enabled (on != 0; enabling clears the totals), every eager evaluation
  * that runs a block-list kernel copies its row tiles' block counts to the
