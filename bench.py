@@ -311,13 +311,20 @@ def bench_property(args):
     pts, w, sizes = atom_product_grids(coords)
     npts = len(pts)
     ms, res = {}, {}
-    for prec in ("mixed", "fp64"):
+    # "mixed_all": the mixed path without the FP64 pass near the nuclei
+    # (radius 0), which on atom-centred grids covers many points
+    radius = ctx.fp64_refined()[0]
+    for prec in ("mixed", "fp64", "mixed_all"):
+        ctx.set_fp64_radius(0.0 if prec == "mixed_all" else radius)
+        p = "mixed" if prec == "mixed_all" else prec
         for _ in range(args.warmup):
-            ctx.properties(pts, w, sizes, coords, precision=prec)
+            ctx.properties(pts, w, sizes, coords, precision=p)
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            res[prec] = ctx.properties(pts, w, sizes, coords, precision=prec)
+            res[prec] = ctx.properties(pts, w, sizes, coords, precision=p)
         ms[prec] = 1e3 * (time.perf_counter() - t0) / args.steps
+        if prec == "mixed":
+            refined = ctx.fp64_refined()[1]
     iso = {p: res[p]["shield"].sum(axis=1)[:, :3].sum(axis=1) / 3.0 for p in res}
     chi = {p: res[p]["magnet"].sum(axis=0)[:3].sum() / 3.0 for p in res}
     ctx.close()
@@ -332,6 +339,13 @@ def bench_property(args):
                                "host arrays in and out", "points_per_step": npts},
         "fp64_gpu": {"value": round(npts / (ms["fp64"] / 1e3), 1), "ms_per_step": round(ms["fp64"], 3)},
         "mixed_over_fp64": round(ms["fp64"] / ms["mixed"], 3),
+        "fp64_near_nuclei": {"radius_bohr": radius, "points_per_step": refined,
+                             "fraction": round(refined / npts, 4)},
+        "mixed_without_fp64_near_nuclei": {
+            "value": round(npts / (ms["mixed_all"] / 1e3), 1), "ms_per_step": round(ms["mixed_all"], 3),
+            "over_fp64": round(ms["fp64"] / ms["mixed_all"], 3),
+            "shielding_vs_fp64_max_abs_ppm": float(np.abs(iso["mixed_all"] - iso["fp64"]).max()),
+            "magnetizability_vs_fp64_rel": float(abs(chi["mixed_all"] - chi["fp64"]) / abs(chi["fp64"]))},
         "results": {"shielding_iso_ppm_fp64": [round(float(iso["fp64"].min()), 4),
                                                round(float(iso["fp64"].max()), 4)],
                     "magnetizability_iso_au_fp64": float(chi["fp64"]),

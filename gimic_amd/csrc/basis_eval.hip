@@ -145,6 +145,14 @@ __device__ __forceinline__ void store(const EvalArgs &a, size_t row, int f, doub
         gr[0] = (float)gx;
         gr[(size_t)tp * lda] = (float)gy;
         gr[2 * (size_t)tp * lda] = (float)gz;
+    } else if (MODE == 4) {
+        // FP64 into the LDS tile: value plane, then the three gradient planes
+        const size_t pl = (size_t)a.tile_points * a.lda;
+        double *d = (double *)basis_tile + row * (size_t)a.lda + f;
+        d[0] = v;
+        d[pl] = gx;
+        d[2 * pl] = gy;
+        d[3 * pl] = gz;
     } else {
         // value as three BF16 parts, x = h + m + l exactly
         const float x = (float)v;
@@ -273,8 +281,14 @@ __global__ __launch_bounds__(256) void basis_eval_kernel(EvalArgs a) {
 // (which is what leaving the block out means), and a shell that is not has
 // all its functions in listed blocks, adjacent in the list, so its
 // functions stay contiguous in the compressed columns.
-template <int LMAX>
+//
+// F64: FP64 values and gradients of dense rows (the FP64 contraction's
+// input), 32 bytes per function in the image; no compression.  Staging
+// keeps the global stores contiguous whatever the function order: a
+// wavefront's shells (ordered by l) need not own adjacent columns.
+template <int LMAX, bool F64 = false>
 __global__ __launch_bounds__(256) void basis_eval_staged_kernel(EvalArgs a) {
+    constexpr int kFunBytes = F64 ? 32 : 18;  // image bytes per function
     const int tid = threadIdx.x, tp = a.tile_points;
     const size_t row0 = (size_t)blockIdx.x * tp;
     int wt = a.lda;  // functions per row of this tile's output
@@ -282,7 +296,7 @@ __global__ __launch_bounds__(256) void basis_eval_staged_kernel(EvalArgs a) {
     __bf16 *phi_dst = a.phis + row0 * 3 * (size_t)a.lda;
     float *grad_dst = a.grad32 + row0 * (size_t)a.lda;
     size_t grad_plane = (size_t)a.gplane;
-    if (a.toff) {
+    if (!F64 && a.toff) {
         const int t = (int)(row0 / a.tile_rows);
         const int na = a.bcount[t];
         if (na == 0) return;  // nothing of this tile is read (uniform per workgroup)
@@ -299,13 +313,14 @@ __global__ __launch_bounds__(256) void basis_eval_staged_kernel(EvalArgs a) {
     }
     // rows per pass: as many as the LDS image holds at this width (at least
     // one: the image holds a dense row)
-    const int rpp = max(1, min(tp, (int)(a.lds_image / ((size_t)wt * 18))));
+    const int rpp = max(1, min(tp, (int)(a.lds_image / ((size_t)wt * kFunBytes))));
     for (int r0 = 0; r0 < tp; r0 += rpp) {
         const int nr = min(rpp, tp - r0);
         EvalArgs b = a;
         b.lda = wt;         // row stride of the LDS image
         b.tile_points = nr; // rows in the image
-        const size_t phi_bytes = (size_t)nr * wt * 6, grad_bytes = (size_t)nr * wt * 12;
+        const size_t phi_bytes = (size_t)nr * wt * (F64 ? 8 : 6),
+                     grad_bytes = (size_t)nr * wt * (F64 ? 24 : 12);
         {
             uint4 *z = (uint4 *)basis_tile;
             const size_t n = (phi_bytes + grad_bytes) / 16;
@@ -330,16 +345,28 @@ __global__ __launch_bounds__(256) void basis_eval_staged_kernel(EvalArgs a) {
             if (sqrt(s) > sh.radius) continue;  // screened: zeros already in place
             double g, h;
             radial_parts(a, sh, s, g, h);
+            constexpr int M = F64 ? 4 : 3;
             if (a.turbomole) {
-                if (a.spherical) dispatch_l<3, true, true, LMAX>(b, lp, sh, rr, g, h);
-                else dispatch_l<3, true, false, LMAX>(b, lp, sh, rr, g, h);
+                if (a.spherical) dispatch_l<M, true, true, LMAX>(b, lp, sh, rr, g, h);
+                else dispatch_l<M, true, false, LMAX>(b, lp, sh, rr, g, h);
             } else {
-                if (a.spherical) dispatch_l<3, false, true, LMAX>(b, lp, sh, rr, g, h);
-                else dispatch_l<3, false, false, LMAX>(b, lp, sh, rr, g, h);
+                if (a.spherical) dispatch_l<M, false, true, LMAX>(b, lp, sh, rr, g, h);
+                else dispatch_l<M, false, false, LMAX>(b, lp, sh, rr, g, h);
             }
         }
         __syncthreads();
         // this pass's rows are contiguous in both outputs
+        if (F64) {
+            const size_t n16 = (size_t)nr * wt / 2;  // 16-byte units per plane
+            for (int d = 0; d < 4; d++) {
+                const uint4 *src = (const uint4 *)(basis_tile + (size_t)d * nr * wt * 8);
+                double *base = d == 0 ? a.phi64 : a.grad64 + (size_t)(d - 1) * a.gplane;
+                uint4 *dst = (uint4 *)(base + (row0 + r0) * (size_t)wt);
+                for (size_t i = tid; i < n16; i += 256) dst[i] = src[i];
+            }
+            __syncthreads();
+            continue;
+        }
         {
             const uint4 *src = (const uint4 *)basis_tile;
             uint4 *dst = (uint4 *)(phi_dst + (size_t)r0 * 3 * wt);
@@ -364,6 +391,30 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
     const long long total = (long long)a.rows * a.nshell;
     const unsigned nblk = (unsigned)((total + 255) / 256);
     if (total <= 0) return hipSuccess;
+    if (mode == 0 && (size_t)a.lda * 32 <= 150 * 1024 && a.lmax >= 0 && a.lmax <= kDevMaxL) {
+        // FP64 rows staged through LDS: the largest tile of 1, 2, 4 or 8 rows
+        // within 72 KiB (or one row up to 150 KiB)
+        EvalArgs b = a;
+        const size_t row_bytes = (size_t)a.lda * 32;
+        int tp = 8;
+        while (tp > 1 && tp * row_bytes > 72 * 1024) tp >>= 1;
+        if (a.rows % tp) return hipErrorInvalidValue;
+        b.tile_points = tp;
+        b.lds_image = tp * row_bytes;
+        b.toff = nullptr;
+        auto launch = [&](auto kernel) -> hipError_t {
+            hipError_t e = hipFuncSetAttribute((const void *)kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)std::max<size_t>(b.lds_image, 72 * 1024));
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(kernel, dim3((unsigned)(a.rows / tp)), dim3(256), b.lds_image, s, b);
+            return hipGetLastError();
+        };
+        if (a.lmax <= 1) return launch(basis_eval_staged_kernel<1, true>);
+        if (a.lmax == 2) return launch(basis_eval_staged_kernel<2, true>);
+        if (a.lmax == 3) return launch(basis_eval_staged_kernel<3, true>);
+        return launch(basis_eval_staged_kernel<5, true>);
+    }
     if (mode == 0)
         hipLaunchKernelGGL(basis_eval_kernel<0>, dim3(nblk), dim3(256), 0, s, a);
     else if (mode == 1)

@@ -5,7 +5,8 @@
 Runs, on cuda:0, (1) a device-to-device copy of exactly 1 GiB (torch), the
 reference for scaling FETCH_SIZE / WRITE_SIZE counters to bytes, then (2) R
 J-tensor evaluations of the first N points of the 128^3 grid around the
-60-atom cage (nbf 1800) in the chosen precision.  With N at most one chunk
+60-atom cage (nbf 1800) in the chosen precision (mixed without the FP64
+pass over points next to a nucleus).  With N at most one chunk
 (642,048 points mixed, 441,472 FP64) every evaluation is one launch each of
 the sort, basis evaluation, contraction and tensor assembly kernels.
 """
@@ -35,6 +36,8 @@ def main():
                        device="cuda")
     out = torch.empty((pts.shape[0], 9), dtype=torch.float64, device="cuda")
     ctx.set_mixed_kernel(a.kernel, a.split)
+    # the counters are for the mixed kernels alone: no FP64 pass near nuclei
+    ctx.set_fp64_radius(0.0)
     # first call: kernel selection (mixed) and buffer allocation
     ctx.jtensor_device(pts, precision=a.precision, out=out)
     for i in range(a.reps):
