@@ -286,8 +286,11 @@ __global__ __launch_bounds__(256) void basis_eval_kernel(EvalArgs a) {
 // input), 32 bytes per function in the image; no compression.  Staging
 // keeps the global stores contiguous whatever the function order: a
 // wavefront's shells (ordered by l) need not own adjacent columns.
+// (four wavefronts per SIMD for l <= 3: the LDS images of four workgroups
+// fit a CU, and the registers are held to 128)
 template <int LMAX, bool F64 = false>
-__global__ __launch_bounds__(256) void basis_eval_staged_kernel(EvalArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LMAX <= 3 ? 4 : 1)))
+void basis_eval_staged_kernel(EvalArgs a) {
     constexpr int kFunBytes = F64 ? 32 : 18;  // image bytes per function
     const int tid = threadIdx.x, tp = a.tile_points;
     const size_t row0 = (size_t)blockIdx.x * tp;
