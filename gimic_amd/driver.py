@@ -50,11 +50,17 @@ def spin_cases(uhf):
 
 def run_integral(kw, ctx, g, b, uhf, precision="fp64", out=sys.stdout):
     jmod = bool(kw["Essential"].get("jmod", False))
-    plane = flux.plane_of(g, b)
+    # product grids are generated on the device from their 1-D rules (the
+    # same points and weights, bit for bit); a file grid goes as host arrays
+    plane = flux.plane_of(g, b) if g.mode == "file" else None
     res = {}
     for spin in spin_cases(uhf):
-        cur, mod, acid = ctx.flux_planes([plane], spin=spin, precision=precision, modulus=True,
-                                         acid=True)
+        if plane is None:
+            cur, mod, acid = ctx.flux_plane_grids([g], [b], spin=spin, precision=precision,
+                                                  modulus=True, acid=True)
+        else:
+            cur, mod, acid = ctx.flux_planes([plane], spin=spin, precision=precision,
+                                             modulus=True, acid=True)
         res[spin] = dict(current=tuple(cur[0]), modulus=tuple(mod[0]),
                          acid=float(np.sqrt(acid[0])))
     if jmod:

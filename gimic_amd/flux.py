@@ -30,12 +30,15 @@ def plane_of(grid, bfield):
 
 def integrate_current(ctx, grid, bfield, spin="total", precision="fp64"):
     """-> (net, positive, negative) flux in au through `grid`"""
-    out = ctx.flux_planes([plane_of(grid, bfield)], spin=spin, precision=precision)
+    out = integrate_planes(ctx, [grid], [bfield], spin=spin, precision=precision)
     return tuple(float(v) for v in out[0])
 
 
 def integrate_planes(ctx, grids, bfields, spin="total", precision="fp64"):
-    """many planes in one call -> (nplanes, 3)"""
+    """many planes in one call -> (nplanes, 3); product grids are generated
+    on the device (identical points and weights), file grids go as arrays"""
+    if all(g.mode != "file" for g in grids):
+        return ctx.flux_plane_grids(grids, bfields, spin=spin, precision=precision)
     return ctx.flux_planes([plane_of(g, b) for g, b in zip(grids, bfields)], spin=spin,
                            precision=precision)
 

@@ -210,3 +210,41 @@ def test_device_plane_grids_equal_host_arrays(precision):
     for g, e in zip(lay, one):
         assert np.allclose(g, e, rtol=1e-12, atol=1e-16), (g, e)
     ctx.close()
+
+
+def test_plane_grid_descriptors_are_checked():
+    """n1 or n2 below 1 is an argument error, reported and not launched; a
+    one-point plane gives w n.(J B) at r = o + p1 v1 + p2 v2 + p3 v3"""
+    from gimic_amd import synth
+    from gimic_amd.lib import ptr
+    s = synth.ring12()
+    ctx = s.context()
+    L = ctx._lib
+    # every array handed to the library is held by a name for the call
+    one = np.ones(1, dtype=np.int32)
+    zero = np.zeros(1, dtype=np.int32)
+    o = np.array([0.3, -0.2, 0.5])
+    ax = np.eye(3).reshape(-1)
+    p3w3 = np.array([0.0, 1.0])
+    x1, w1 = np.array([0.25]), np.array([0.5])
+    x2, w2 = np.array([-0.5]), np.array([2.0])
+    nz = np.array([0.0, 0.0, 1.0])
+    rad = np.array([1e10])
+    out = np.zeros(3)
+
+    def call(n1, n2):
+        return L.gimic_flux_plane_grids(ctx._h, 1, ptr(n1), ptr(n2), ptr(o), ptr(ax), ptr(p3w3),
+                                        ptr(x1), ptr(w1), ptr(x2), ptr(w2), ptr(nz), ptr(nz),
+                                        ptr(o), ptr(rad), 0, 0, ptr(out), None, None)
+
+    for n1, n2 in ((zero, one), (one, zero)):
+        assert call(n1, n2) != 0
+        assert b"n1, n2" in L.gimic_last_error()
+    print("case single-point plane", flush=True)
+    assert call(one, one) == 0
+    r = o + np.array([0.25, -0.5, 0.0])
+    J = ctx.jtensor(r[None], precision="fp64")[0].reshape(3, 3)  # [b, m]
+    expect = (1.0 * 2.0) * 0.5 * (J[2] @ nz)  # w = (w3 w2) w1, n.(J B) with B = n = z
+    assert np.isclose(out[0], expect, rtol=1e-13, atol=1e-300)
+    assert out[1] + out[2] == out[0]
+    ctx.close()

@@ -124,3 +124,23 @@ def test_flux_with_refinement_eager_equals_graph_and_matches_its_tensors():
     ctx.set_graph(False)
     assert np.array_equal(f1g[0], f1)
     ctx.close()
+
+
+def test_radius_covering_everything_gives_fp64_and_zero_radius_nothing():
+    from gimic_amd import synth
+    s = synth.ring12()
+    ctx = s.context()
+    pts = _points_near_and_far(np.asarray(s.coords), np.random.default_rng(8), 700, 100)
+    print("case refine radius 1e3", flush=True)
+    j64 = ctx.jtensor(pts, precision="fp64")
+    ctx.set_fp64_radius(1e3)
+    jall = ctx.jtensor(pts, precision="mixed")
+    assert ctx.fp64_refined() == (1e3, len(pts))
+    scale = np.abs(j64).max(axis=1, keepdims=True) + 1e-300
+    assert np.all(np.abs(jall - j64) <= 1e-12 * scale)
+    # points far from every nucleus: nothing selected
+    ctx.set_fp64_radius(0.4)
+    far = np.asarray(s.coords).max(0) + 20.0 + np.random.default_rng(2).uniform(0, 1, (50, 3))
+    ctx.jtensor(far, precision="mixed")
+    assert ctx.fp64_refined() == (0.4, 0)
+    ctx.close()
