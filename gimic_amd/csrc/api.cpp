@@ -229,7 +229,7 @@ struct gimic_ctx {
     DevBuf blk_beg, blk_sh;
     // spatial tiles of a chunk: sorted points, their caller indices, sort
     // scratch, and the screening block list of every contraction row tile
-    DevBuf spts, perm, bpart, sortscr, bcount, blist, toff;
+    DevBuf spts, perm, bpart, sortscr, bcount, blist, toff, k8count, k8list, bzero;
     // property quadrature (gimic_ctx_properties): points + weights, tensors,
     // partition offsets, nuclei, sums
     DevBuf prop_in, prop_jt, prop_offs, prop_nuc, prop_out;
@@ -257,7 +257,7 @@ struct gimic_ctx {
     int mixed_split = 0;  // N-split count of the pre-split kernel (0: by workgroup count)
     // block-list statistics (gimic_ctx_set_tile_stats): tiles, sum n, sum n^2, rows per tile
     bool tile_stats = false;
-    double tile_acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double tile_acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     float tune_ms[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     std::vector<std::pair<int, float>> tune_log;  // (variant * 100 + split, ms)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
@@ -477,11 +477,11 @@ gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec)
                                  hipMemcpyHostToDevice, st),
                   "hipMemcpyAsync");
         // the same FP32 values as three BF16 parts (x = h + m + l exactly),
-        // one block per (set, N-tile, K-step) laid out as the pre-split
-        // kernel's LDS buffer: [part][k-group g][column][8 k], column =
-        // matrix * 32 + function within the N-tile
-        const size_t nks = ld / 32, ntl = nc / 32, blk = kMx2BlockElems;
-        std::vector<uint16_t> bs(spins.size() * ntl * nks * blk, 0);
+        // one 8-function group per (set, N-tile, group kg), [part][column][8 k],
+        // column = matrix * 32 + function within the N-tile; the pre-split
+        // kernels gather four groups per K-step into their LDS buffers
+        const size_t nks = ld / 32, ntl = nc / 32, grp = kMx2GroupElems;
+        std::vector<uint16_t> bs(spins.size() * ntl * nks * kMx2StepElems + 1024, 0);
         auto bf = [](float x) {
             uint32_t u;
             std::memcpy(&u, &x, 4);
@@ -494,26 +494,25 @@ gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec)
             std::memcpy(&x, &u, 4);
             return x;
         };
-        const size_t plane = 4 * (size_t)kMx2Cols * 8;
+        const size_t part = (size_t)kMx2Cols * 8;
         for (size_t si = 0; si < spins.size(); si++)
             for (size_t nt = 0; nt < ntl; nt++)
                 for (int col = 0; col < kMx2Cols; col++) {
                     const size_t mat = col / 32, nu = nt * 32 + col % 32;
                     const float *src = &b32[si * setsz + mat * matsz + nu * ld];
-                    for (size_t ks = 0; ks < nks; ks++) {
-                        uint16_t *dst = &bs[((si * ntl + nt) * nks + ks) * blk];
-                        for (int gk = 0; gk < 4; gk++)
-                            for (int e = 0; e < 8; e++) {
-                                const float x = src[ks * 32 + gk * 8 + e];
-                                const uint16_t h = bf(x);
-                                const float r1 = x - fl(h);
-                                const uint16_t mm = bf(r1);
-                                const uint16_t l = bf(r1 - fl(mm));
-                                const size_t o = ((size_t)gk * kMx2Cols + col) * 8 + e;
-                                dst[o] = h;
-                                dst[plane + o] = mm;
-                                dst[2 * plane + o] = l;
-                            }
+                    for (size_t kg = 0; kg < 4 * nks; kg++) {
+                        uint16_t *dst = &bs[((si * ntl + nt) * 4 * nks + kg) * grp];
+                        for (int e = 0; e < 8; e++) {
+                            const float x = src[kg * 8 + e];
+                            const uint16_t h = bf(x);
+                            const float r1 = x - fl(h);
+                            const uint16_t mm = bf(r1);
+                            const uint16_t l = bf(r1 - fl(mm));
+                            const size_t o = (size_t)col * 8 + e;
+                            dst[o] = h;
+                            dst[part + o] = mm;
+                            dst[2 * part + o] = l;
+                        }
                     }
                 }
         m->bs3.alloc(bs.size() * 2, st, false);
@@ -627,6 +626,9 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
     grow(c->bcount, rows_cap / 128 * 4, false);
     grow(c->blist, rows_cap / 128 * nblk * 4, false);
     grow(c->toff, (rows_cap / 128 + 1) * 4, false);
+    grow(c->k8count, rows_cap / 128 * 4, false);
+    grow(c->k8list, rows_cap / 128 * nblk * 4 * 4, false);
+    grow(c->bzero, 12 * 1024, true);  // stays zero: never written
     const int nmatset = M.nmatset;
     for (size_t start = 0; start < (size_t)npts; start += chunk) {
         const int n_c = (int)std::min(chunk, (size_t)npts - start);
@@ -675,6 +677,9 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         ca.bcount = c->bcount.as<int>();
         ca.blist = c->blist.as<int>();
         ca.nblk = nblk;
+        ca.k8count = c->k8count.as<int>();
+        ca.k8list = c->k8list.as<int>();
+        ca.bzero = c->bzero.as<__bf16>();
         // screening block lists for row tiles of tile_rows rows
         auto lists = [&](int tile_rows) {
             TileArgs ta{};
@@ -689,18 +694,25 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             ta.coords = c->coords.as<double>();
             ta.bcount = c->bcount.as<int>();
             ta.blist = c->blist.as<int>();
+            ta.k8count = c->k8count.as<int>();
+            ta.k8list = c->k8list.as<int>();
             hip_check(launch_block_lists(ta, st), "block_lists");
             if (c->tile_stats && mode == EvalMode::Eager) {
                 const int nt = rows / tile_rows;
-                std::vector<int> cnt(nt);
+                std::vector<int> cnt(nt), cnt8(nt);
                 hip_check(hipMemcpyAsync(cnt.data(), ta.bcount, nt * sizeof(int),
                                          hipMemcpyDeviceToHost, st),
                           "hipMemcpyAsync");
+                hip_check(hipMemcpyAsync(cnt8.data(), ta.k8count, nt * sizeof(int),
+                                         hipMemcpyDeviceToHost, st),
+                          "hipMemcpyAsync");
                 hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-                for (int k : cnt) {
+                for (int t = 0; t < nt; t++) {
+                    const int k = cnt[t];
                     c->tile_acc[0] += 1.0;
                     c->tile_acc[1] += k;
                     c->tile_acc[2] += (double)k * k;
+                    c->tile_acc[4] += (double)k * cnt8[t] / 4.0;
                 }
                 c->tile_acc[3] = tile_rows;
             }
@@ -1160,7 +1172,8 @@ void flux_run(gimic_ctx *c, int64_t npts, int nplanes, const int64_t *offs, cons
             (uintptr_t)M.b32.p, (uintptr_t)c->phis.p, (uintptr_t)M.bs3.p,
             (uintptr_t)c->mixed_split, (uintptr_t)c->spts.p, (uintptr_t)c->perm.p,
             (uintptr_t)c->bpart.p, (uintptr_t)c->sortscr.p, (uintptr_t)c->bcount.p,
-            (uintptr_t)c->blist.p, (uintptr_t)c->toff.p, (uintptr_t)refine};
+            (uintptr_t)c->blist.p, (uintptr_t)c->toff.p, (uintptr_t)c->k8count.p,
+            (uintptr_t)c->k8list.p, (uintptr_t)c->bzero.p, (uintptr_t)refine};
         key.insert(key.end(), source.begin(), source.end());
         if (!c->graph_exec || key != c->graph_key) {
             if (c->graph_exec) {
@@ -1373,6 +1386,7 @@ int gimic_ctx_tile_stats(const gimic_ctx *c, double *out) {
         if (!c || !out) throw Error(GIMIC_EINVAL, "null argument");
         for (int k = 0; k < 4; k++) out[k] = c->tile_acc[k];
         out[4] = c->lda / 32;
+        out[5] = c->tile_acc[4];
     });
 }
 

@@ -8,11 +8,11 @@
 //   * the basis kernel writes the left operand already split, [row][k/8][3][8]
 //     (48 contiguous bytes per lane and K-step), and
 //   * the right operand (constant per density) is split once on the host into
-//     44 KiB blocks per (density set, N-tile, K-step) that are byte images of
-//     the kernel's LDS buffer, [plane][k-group][column][8].
-// The K-loop therefore has no conversion work: each K-step streams a later
-// block into a free LDS buffer with gfx950's direct global-to-LDS loads
-// (global_load_lds_dwordx4: no VGPRs, no ds_write), loads the next A
+//     10.5 KiB groups per (density set, N-tile, 8-function group),
+//     [part][column][8]; a K-step's 44 KiB LDS buffer holds four of them.
+// The K-loop therefore has no conversion work: each K-step streams the four
+// groups of a later step into a free LDS buffer with gfx950's direct
+// global-to-LDS loads (global_load_lds_dwordx4: no VGPRs, no ds_write), loads the next A
 // fragments, and runs 168 MFMAs per wavefront against B fragments that are
 // read from LDS one column block ahead.  NBUF = 2 streams block ks+1 during
 // K-step ks; NBUF = 3 streams block ks+2 (154 KiB of LDS), so a block has
@@ -38,10 +38,15 @@
 
 //
 // Screening.  A workgroup visits only the 32-function blocks listed for its
-// row tile (tiles.hip): the K-loop walks the list, and the listed N-blocks
-// are split evenly over the nsplit workgroups of the row tile.  The LDS
-// buffers rotate over list positions, so the pipeline is the same as for a
-// dense walk.
+// row tile (tiles.hip): the listed N-blocks are split evenly over the nsplit
+// workgroups of the row tile, and the K-loop walks the live 8-function
+// groups of the listed blocks, four per K-step (an MFMA's K of 32): groups
+// in which screening zeroes every function of the tile are skipped as
+// well, which removes a fifth of the K-steps on the 60-atom cage.  Each
+// lane group of a wavefront reads its own group's A fragment (the group's
+// place in the compressed row), and the buffer copy gathers the four
+// groups.  The LDS buffers rotate over K-steps, so the pipeline is the same
+// as for a dense walk.
 //
 // Workgroup order.  Every workgroup re-reads its rows of A once per N-tile,
 // and every row tile streams the B blocks of its listed (N, K) pairs.  The
@@ -64,7 +69,6 @@ constexpr int kNu = 32;
 constexpr int kCols = kMx2Cols;             // 224
 constexpr int kCB = kCols / 16;             // 14 column blocks
 constexpr int kBlk = kMx2BlockElems;        // BF16 per block / LDS buffer
-constexpr int kPlane = 4 * kCols * 8;       // BF16 per part plane of a block
 constexpr int kPieces = kBlk * 2 / 1024;    // 1 KiB pieces per block (44)
 constexpr size_t kBufBytes = (size_t)kBlk * 2;
 // pieces a wavefront copies of each block: wavefront w copies w, w + WAVES, ...
@@ -73,28 +77,41 @@ constexpr int pieces_max() { return (kPieces + WAVES - 1) / WAVES; }
 __device__ __forceinline__ int pieces_of(int wave, int waves) {
     return (kPieces - wave + waves - 1) / waves;
 }
-static_assert(3 * kPlane <= kBlk, "block too small");
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// piece i of this wavefront's share of a 44 KiB block (64 lanes x 16 bytes
-// per instruction, LDS base in M0)
+// A K-step's LDS buffer is [group slot j][part][column][8]: slot j holds one
+// 8-function group (10752 bytes, contiguous in HBM too), so its 44 pieces are
+// 11 per slot, the eleventh a half piece (lanes 0-31)
+constexpr int kGroupBytes = kMx2GroupElems * 2;   // 10752
+constexpr int kPartElems = kCols * 8;             // BF16 per part of a group
+static_assert(4 * kGroupBytes <= kBlk * 2 && kPieces == 44 && kGroupBytes == 10 * 1024 + 512,
+              "piece layout");
+
+// piece i of this wavefront's share of a K-step's buffer (64 lanes x 16
+// bytes per instruction, LDS base in M0): piece q = 11 j + t copies bytes
+// [1024 t, 1024 t + 1024) of group slot j from global group ent[j] >> 12 of
+// the N-tile (ent[j] < 0: from the zero buffer)
 template <int WAVES>
-__device__ __forceinline__ void stage_piece(const __bf16 *blk, unsigned char *buf, int wave,
-                                            int lane, int i) {
+__device__ __forceinline__ void stage_piece_k8(const __bf16 *Bnt, int4 ent, const __bf16 *bzero,
+                                               unsigned char *buf, int wave, int lane, int i) {
     const int piece = wave + WAVES * i;
-    const unsigned char *src = (const unsigned char *)blk + piece * 1024 + lane * 16;
-    __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(buf + piece * 1024), 16, 0,
-                                     0);
+    const int j = piece / 11, t = piece - 11 * j;
+    const int v = j == 0 ? ent.x : j == 1 ? ent.y : j == 2 ? ent.z : ent.w;
+    const __bf16 *grp = v < 0 ? bzero : Bnt + (size_t)(v >> 12) * kMx2GroupElems;
+    const unsigned char *src = (const unsigned char *)grp + t * 1024 + lane * 16;
+    if (t < 10 || lane < 32)
+        __builtin_amdgcn_global_load_lds((const void *)src,
+                                         (lds_void *)(buf + j * kGroupBytes + t * 1024), 16, 0, 0);
 }
 
 template <int WAVES>
-__device__ __forceinline__ void stage_block(const __bf16 *blk, unsigned char *buf, int wave,
-                                            int lane) {
+__device__ __forceinline__ void stage_block_k8(const __bf16 *Bnt, int4 ent, const __bf16 *bzero,
+                                               unsigned char *buf, int wave, int lane) {
     const int np = pieces_of(wave, WAVES);
 #pragma unroll
     for (int i = 0; i < pieces_max<WAVES>(); i++)
-        if (i < np) stage_piece<WAVES>(blk, buf, wave, lane, i);
+        if (i < np) stage_piece_k8<WAVES>(Bnt, ent, bzero, buf, wave, lane, i);
 }
 
 __device__ __forceinline__ v4f mfma6(const v8bf (&a)[3], const v8bf (&b)[3], v4f acc) {
@@ -149,6 +166,11 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
     const size_t rows = (size_t)a.rows;
     const int *L = a.blist + (size_t)rtile * a.nblk;
     const int na = a.bcount[rtile];  // listed blocks
+    // K-steps of four live 8-function groups of the listed blocks: entry =
+    // (global group << 12) | position of the group in this tile's
+    // compressed rows; -1 pads the last step (zero right operand)
+    const int *K8 = a.k8list + (size_t)rtile * a.nblk * 4;
+    const int nq = (a.k8count[rtile] + 3) >> 2;
     // This tile's basis rows: dense [row][lda], or compressed to the listed
     // blocks (a.toff non-null: the basis kernel wrote only those, so A and
     // the gradients are addressed by list position)
@@ -170,41 +192,47 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
         for (int d = 0; d < 3; d++) rps[3 * i + d] = valid ? (float)a.pts[3 * p + d] : 0.f;
     }
 
-    // lane (g, c) of row block rb: row wave*32 + rb*16 + c, k = 32 ks + 8 g .. + 7
-    const __bf16 *Aw = Atile + (size_t)(wave * 16 * kRB + c) * arow + g * 24;
-    const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kBlk;
+    // lane (g, c) of row block rb: row wave*32 + rb*16 + c, k-group entry g of the K-step
+    const __bf16 *Arow = Atile + (size_t)(wave * 16 * kRB + c) * arow;
+    auto a_off = [&](int4 v) -> size_t {  // this lane's group of a K-step, in BF16 of a row
+        const int x = g == 0 ? v.x : g == 1 ? v.y : g == 2 ? v.z : v.w;
+        return x < 0 ? 0 : (size_t)(cmp ? (x & 4095) : (x >> 12)) * 24;
+    };
+    const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kMx2StepElems;
     const int e0 = (int)((long long)split * na / a.nsplit);
     const int e1 = (int)((long long)(split + 1) * na / a.nsplit);
 
     for (int e = e0; e < e1; e++) {
         const int nt = L[e];
-        const __bf16 *Bnt = Bset + (size_t)nt * nks * kBlk;
+        const __bf16 *Bnt = Bset + (size_t)nt * nks * kMx2StepElems;
         v4f acc[kRB][kCB];
 #pragma unroll
         for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
             for (int cb = 0; cb < kCB; cb++) acc[rb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
 
-        stage_block<WAVES>(Bnt + (size_t)L[0] * kBlk, bbuf, wave, lane);
-        if (NBUF == 3 && na > 1)
-            stage_block<WAVES>(Bnt + (size_t)L[1] * kBlk, bbuf + kBufBytes, wave, lane);
+        const int4 q0 = *(const int4 *)K8, q1 = *(const int4 *)(K8 + (nq > 1 ? 4 : 0));
+        stage_block_k8<WAVES>(Bnt, q0, a.bzero, bbuf, wave, lane);
+        if (NBUF == 3 && nq > 1) stage_block_k8<WAVES>(Bnt, q1, a.bzero, bbuf + kBufBytes, wave, lane);
         v8bf an[kRB][3];
+        {
+            const size_t ao = a_off(q0);
 #pragma unroll
-        for (int rb = 0; rb < kRB; rb++)
+            for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
-            for (int p = 0; p < 3; p++)
-                an[rb][p] = *(const v8bf *)(Aw + rb * 16 * arow + (size_t)(cmp ? 0 : L[0]) * 96 + p * 8);
+                for (int p = 0; p < 3; p++) an[rb][p] = *(const v8bf *)(Arow + rb * 16 * arow + ao + p * 8);
+        }
         __builtin_amdgcn_s_waitcnt(kWaitAll);
         __syncthreads();
 
-        // blocks of the next step's A fragments and of the next step's copy,
+        // groups of the next step's A fragments and of the next step's copy,
         // read one K-step early (the end-of-step wait covers the scalar loads)
-        int blk_a = L[na > 1 ? 1 : 0];
-        int blk_c = L[min(NBUF - 1, na - 1)];
-        for (int ks = 0; ks < na; ks++) {
-            const int a_blk = blk_a, c_blk = blk_c;
-            blk_a = L[min(ks + 2, na - 1)];
-            blk_c = L[min(ks + NBUF, na - 1)];
+        int4 ent_a = q1;
+        int4 ent_c = *(const int4 *)(K8 + 4 * min(NBUF - 1, nq - 1));
+        for (int ks = 0; ks < nq; ks++) {
+            const int4 ea = ent_a, ec = ent_c;
+            ent_a = *(const int4 *)(K8 + 4 * min(ks + 2, nq - 1));
+            ent_c = *(const int4 *)(K8 + 4 * min(ks + NBUF, nq - 1));
             v8bf as[kRB][3];
 #pragma unroll
             for (int rb = 0; rb < kRB; rb++)
@@ -222,38 +250,37 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
             // works through them.  With NBUF = 3 the copy comes after all A
             // loads (column blocks 3 to 13), so that waiting for "all but the
             // last copy" covers the A loads.
-            const bool more = ks + 1 < na;
+            const bool more = ks + 1 < nq;
             constexpr int kALoads = 3 * kRB;
             constexpr int kCopyFrom = NBUF == 3 ? kCB - kPW : 0;
             static_assert(NBUF != 3 || (kALoads + 1) / 2 <= kCopyFrom, "A loads must precede the copy");
             const int ahead = ks + NBUF - 1;
-            const bool copy = NBUF == 3 || ahead < na;
-            const __bf16 *cblk = Bnt + (size_t)c_blk * kBlk;
+            const bool copy = NBUF == 3 || ahead < nq;
+            const size_t aoff = a_off(ea);
             unsigned char *cbuf = bbuf + (size_t)(ahead % NBUF) * kBufBytes;
             const __bf16 *cur = (const __bf16 *)(bbuf + (size_t)(ks % NBUF) * kBufBytes);
             v8bf bf[2][3];
             {
-                const int off = (g * kCols + c) * 8;
+                const int off = g * kMx2GroupElems + c * 8;
 #pragma unroll
-                for (int p = 0; p < 3; p++) bf[0][p] = *(const v8bf *)(cur + p * kPlane + off);
+                for (int p = 0; p < 3; p++) bf[0][p] = *(const v8bf *)(cur + p * kPartElems + off);
             }
 #pragma unroll
             for (int cb = 0; cb < kCB; cb++) {
                 const int sl = cb & 1;
                 if (cb + 1 < kCB) {
-                    const int off = (g * kCols + (cb + 1) * 16 + c) * 8;
+                    const int off = g * kMx2GroupElems + ((cb + 1) * 16 + c) * 8;
 #pragma unroll
                     for (int p = 0; p < 3; p++)
-                        bf[sl ^ 1][p] = *(const v8bf *)(cur + p * kPlane + off);
+                        bf[sl ^ 1][p] = *(const v8bf *)(cur + p * kPartElems + off);
                 }
 #pragma unroll
                 for (int j = 2 * cb; j < 2 * cb + 2; j++)
                     if (more && j < kALoads)
-                        an[j / 3][j % 3] = *(const v8bf *)(Aw + (j / 3) * 16 * arow +
-                                                          (size_t)(cmp ? ks + 1 : a_blk) * 96 +
-                                                          (j % 3) * 8);
+                        an[j / 3][j % 3] =
+                            *(const v8bf *)(Arow + (j / 3) * 16 * arow + aoff + (j % 3) * 8);
                 if (copy && cb >= kCopyFrom && cb - kCopyFrom < np)
-                    stage_piece<WAVES>(cblk, cbuf, wave, lane, cb - kCopyFrom);
+                    stage_piece_k8<WAVES>(Bnt, ec, a.bzero, cbuf, wave, lane, cb - kCopyFrom);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int rb = 0; rb < kRB; rb++) acc[rb][cb] = mfma6(as[rb], bf[sl], acc[rb][cb]);

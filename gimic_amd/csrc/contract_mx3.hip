@@ -67,7 +67,8 @@ __device__ __forceinline__ void read_group(v8bf (&bf)[7], const __bf16 *cur, int
                                            int c) {
 #pragma unroll
     for (int m = 0; m < 7; m++)
-        bf[m] = *(const v8bf *)(cur + pb * kPlane + ((2 * h + q) * kCols + m * 32 + c) * 8);
+        bf[m] = *(const v8bf *)(cur + (2 * h + q) * kMx2GroupElems + pb * (kCols * 8) +
+                                (m * 32 + c) * 8);
 }
 
 // one butterfly step: lanes with bit W set keep the upper W values, the
@@ -125,12 +126,12 @@ __global__ __launch_bounds__(256, 1) void contract_mx3_kernel(ContractArgs a) {
 
     // A row wave*32 + c; k-group q of each half
     const __bf16 *Aw = a.as3 + (row0 + wave * 32 + c) * arow + q * 24;
-    const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kBlk;
+    const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kMx2StepElems;
     const int nt0 = split * a.tiles_per_split;
     const int nt1 = min(nt0 + a.tiles_per_split, a.ntiles);
 
     for (int nt = nt0; nt < nt1; nt++) {
-        const __bf16 *Bnt = Bset + (size_t)nt * nks * kBlk;
+        const __bf16 *Bnt = Bset + (size_t)nt * nks * kMx2StepElems;
         v16f acc[7];
 #pragma unroll
         for (int m = 0; m < 7; m++)
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(256, 1) void contract_mx3_kernel(ContractArgs a) {
             for (int e = 0; e < 16; e++) acc[m][e] = 0.f;
 
         stage_block(Bnt, bbuf, wave, lane);
-        if (NBUF == 3 && nks > 1) stage_block(Bnt + kBlk, bbuf + kBufBytes, wave, lane);
+        if (NBUF == 3 && nks > 1) stage_block(Bnt + kMx2StepElems, bbuf + kBufBytes, wave, lane);
         v8bf an[2][3];
 #pragma unroll
         for (int h = 0; h < 2; h++)
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(256, 1) void contract_mx3_kernel(ContractArgs a) {
             const bool more = ks + 1 < nks;
             const int ahead = ks + NBUF - 1;
             const bool copy = NBUF == 3 || ahead < nks;
-            const __bf16 *cblk = Bnt + (size_t)min(ahead, nks - 1) * kBlk;
+            const __bf16 *cblk = Bnt + (size_t)min(ahead, nks - 1) * kMx2StepElems;
             unsigned char *cbuf = bbuf + (size_t)(ahead % NBUF) * kBufBytes;
             const __bf16 *cur = (const __bf16 *)(bbuf + (size_t)(ks % NBUF) * kBufBytes);
 
@@ -285,14 +286,14 @@ __global__ __launch_bounds__(256, 1) void contract_mx3p_kernel(ContractArgs a) {
     for (int i = tid; i < kRows * kNQ; i += 256) qs[i] = 0.0;
 
     const __bf16 *Aw = a.as3 + (row0 + wave * 32 + c) * arow + q * 24;
-    const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kBlk;
+    const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kMx2StepElems;
     const int nt0 = split * a.tiles_per_split;
     const int nt1 = min(nt0 + a.tiles_per_split, a.ntiles);
     // vmcnt(6): all but the six A loads issued after the last copy
     constexpr int kWaitAllButA = 6 | (7 << 4) | (15 << 8);
 
     for (int nt = nt0; nt < nt1; nt++) {
-        const __bf16 *Bnt = Bset + (size_t)nt * nks * kBlk;
+        const __bf16 *Bnt = Bset + (size_t)nt * nks * kMx2StepElems;
         v16f acc[7];
 #pragma unroll
         for (int m = 0; m < 7; m++)
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(256, 1) void contract_mx3p_kernel(ContractArgs a) {
             for (int e = 0; e < 16; e++) acc[m][e] = 0.f;
 
         stage_block(Bnt, bbuf, wave, lane);
-        stage_block(Bnt + (size_t)min(1, nks - 1) * kBlk, bbuf + kBufBytes, wave, lane);
+        stage_block(Bnt + (size_t)min(1, nks - 1) * kMx2StepElems, bbuf + kBufBytes, wave, lane);
         v8bf an[2][3];
 #pragma unroll
         for (int h = 0; h < 2; h++)
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(256, 1) void contract_mx3p_kernel(ContractArgs a) {
                 for (int p = 0; p < 3; p++) as[h][p] = an[h][p];
             const __bf16 *cur = (const __bf16 *)(bbuf + (size_t)(ks % 3) * kBufBytes);
             const __bf16 *nxt = (const __bf16 *)(bbuf + (size_t)((ks + 1) % 3) * kBufBytes);
-            const __bf16 *cblk = Bnt + (size_t)min(ks + 2, nks - 1) * kBlk;
+            const __bf16 *cblk = Bnt + (size_t)min(ks + 2, nks - 1) * kMx2StepElems;
             unsigned char *cbuf = bbuf + (size_t)((ks + 2) % 3) * kBufBytes;
             const size_t ka = (size_t)min(ks + 1, nks - 1);
 #pragma unroll

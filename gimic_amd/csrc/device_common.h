@@ -127,6 +127,10 @@ struct ContractArgs {
     // a tile takes the listed N-blocks [s * count / nsplit, (s+1) * count / nsplit)
     const int *bcount, *blist;
     int nblk;
+    // pre-split kernels: K-steps of four listed 8-function groups (TileArgs)
+    // and a zeroed 10752-byte buffer standing in for the padding groups
+    const int *k8count, *k8list;
+    const __bf16 *bzero;
     // compressed operands (pre-split kernels; see EvalArgs): null for dense
     const int *toff;
 };
@@ -145,7 +149,13 @@ hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, int wav
 // Pre-split operands on 32x32x16 MFMA blocks, 128 rows per workgroup (contract_mx3.hip)
 hipError_t launch_contract_mx3(const ContractArgs &a, int nbuf, hipStream_t s);
 constexpr int kMx2Cols = 224;                         // 7 matrices x 32 functions
-constexpr int kMx2BlockElems = 22528;                 // 3*4*224*8 = 21504, padded to 44 KiB
+constexpr int kMx2BlockElems = 22528;                 // LDS buffer: 3*4*224*8 = 21504, padded to 44 KiB
+// pre-split right operand in HBM: per (density set, N-tile, 8-function group
+// kg) one group of 3 parts x 224 columns x 8 (10752 bytes), [part][column][8];
+// the four groups of a 32-function K-block are consecutive (a K-step image,
+// [group][part][column][8], 43008 bytes), plus 2 KiB of slack at the end
+constexpr int kMx2GroupElems = 3 * kMx2Cols * 8;      // 5376
+constexpr int kMx2StepElems = 4 * kMx2GroupElems;     // 21504
 
 struct FinalizeArgs {
     const double *part;  // [nmatset][nsplit][kNQ][rows], or [nmatset][nsplit][rows][kNQ] if part_rows
@@ -223,6 +233,11 @@ struct TileArgs {
     const DevShell *shells;
     const double *coords;
     int *bcount, *blist;  // [rows / tile_rows], [rows / tile_rows][nblk]
+    // 8-function groups of the listed blocks that screening leaves non-zero:
+    // k8count[t] groups, k8list[t * 4 nblk ...] = (global group << 12) |
+    // (4 * list position of its block + group within the block), padded
+    // with -1 to a multiple of 4 (nblk <= 1024)
+    int *k8count, *k8list;
 };
 hipError_t launch_block_lists(const TileArgs &a, hipStream_t s);
 // toff[t] = sum of bcount[0 .. t-1] for t = 0 .. ntiles (one workgroup)

@@ -189,9 +189,12 @@ __global__ __launch_bounds__(256) void gather_points_kernel(const double *pts, c
 
 // One workgroup per row tile: bounding box of its valid rows, then the
 // ascending list of blocks with a function that may be non-zero there.
+constexpr int kMaxBlk = 1024;  // 32-function blocks per row (32768 functions)
+
 __global__ __launch_bounds__(256) void block_list_kernel(TileArgs a) {
     __shared__ double s[6][256];
     __shared__ int wsum[4];
+    __shared__ unsigned char gmask[kMaxBlk];  // live 8-function groups of each block
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const long long r0 = (long long)blockIdx.x * a.tile_rows;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -228,8 +231,9 @@ __global__ __launch_bounds__(256) void block_list_kernel(TileArgs a) {
     for (int b0 = 0; b0 < a.nblk; b0 += 256) {
         const int b = b0 + t;
         bool act = false;
+        unsigned m4 = 0;  // groups of 8 functions that a live shell touches
         if (any && b < a.nblk) {
-            for (int k = a.blk_beg[b]; k < a.blk_beg[b + 1] && !act; k++) {
+            for (int k = a.blk_beg[b]; k < a.blk_beg[b + 1] && m4 != 15u; k++) {
                 const DevShell sh = a.shells[a.blk_sh[k]];
                 const double *R = a.coords + 3 * sh.atom;
                 double d2 = 0.0;
@@ -239,9 +243,15 @@ __global__ __launch_bounds__(256) void block_list_kernel(TileArgs a) {
                     d2 += e * e;
                 }
                 // relative slack for rounding in the basis kernel's own distance
-                act = sqrt(d2) <= sh.radius * (1.0 + 1e-12) + 1e-12;
+                if (sqrt(d2) <= sh.radius * (1.0 + 1e-12) + 1e-12) {
+                    const int f0 = max(0, sh.fout - 32 * b);
+                    const int f1 = min(31, sh.fout + sh.ncomp_out - 1 - 32 * b);
+                    for (int gq = f0 >> 3; gq <= (f1 >> 3); gq++) m4 |= 1u << gq;
+                }
             }
+            act = m4 != 0u;
         }
+        if (b < a.nblk) gmask[b] = (unsigned char)m4;
         const unsigned long long m = __ballot(act);
         if (lane == 0) wsum[wave] = __popcll(m);
         __syncthreads();
@@ -252,6 +262,21 @@ __global__ __launch_bounds__(256) void block_list_kernel(TileArgs a) {
         __syncthreads();
     }
     if (t == 0) a.bcount[blockIdx.x] = base;
+    // the live groups in list order, four per K-step (the last padded with -1)
+    __syncthreads();
+    if (t == 0) {
+        int *kl = a.k8list + (long long)blockIdx.x * a.nblk * 4;
+        int k = 0, pos = 0;
+        for (int b = 0; b < a.nblk; b++) {
+            const unsigned m = gmask[b];
+            if (!m) continue;
+            for (int gq = 0; gq < 4; gq++)
+                if (m >> gq & 1u) kl[k++] = ((4 * b + gq) << 12) | (4 * pos + gq);
+            pos++;
+        }
+        a.k8count[blockIdx.x] = k;
+        while (k & 3) kl[k++] = -1;
+    }
 }
 
 // exclusive prefix sum of the tiles' block counts, toff[0 .. n]
@@ -317,7 +342,8 @@ hipError_t launch_sort_points(const double *pts, int n, double *spts, int *perm,
 }
 
 hipError_t launch_block_lists(const TileArgs &a, hipStream_t s) {
-    if (a.tile_rows <= 0 || a.rows % a.tile_rows || a.nblk <= 0) return hipErrorInvalidValue;
+    if (a.tile_rows <= 0 || a.rows % a.tile_rows || a.nblk <= 0 || a.nblk > kMaxBlk)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(block_list_kernel, dim3(a.rows / a.tile_rows), dim3(256), 0, s, a);
     return hipGetLastError();
 }

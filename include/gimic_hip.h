@@ -165,10 +165,13 @@ int gimic_ctx_fp64_refined(const gimic_ctx *ctx, double *radius, int64_t *npts);
  * enabled (on != 0; enabling clears the totals), every eager evaluation
  * that runs a block-list kernel copies its row tiles' block counts to the
  * host (a synchronising copy: use it outside timed regions) and adds them
- * up.  out[5]: row tiles, sum of listed blocks, sum of squared listed
- * blocks, rows per tile, blocks per dense row (lda / 32).  A tile of R rows
- * with n listed blocks costs 2 * R * 7 * (32 n)^2 multiply-adds x 2 flops
- * per density set. */
+ * up.  out[6]: row tiles, sum of listed blocks, sum of squared listed
+ * blocks, rows per tile, blocks per dense row (lda / 32), and the sum over
+ * tiles of listed blocks x live 8-function groups / 4.  A tile of R rows
+ * with n listed blocks costs 2 * R * 7 * (32 n)^2 flops per density set
+ * when its K dimension walks whole blocks (the FP64 kernel), and
+ * 2 * R * 7 * 32 n * 8 k with k live groups (the pre-split mixed kernels,
+ * whose K-steps take four live groups). */
 int gimic_ctx_set_tile_stats(gimic_ctx *ctx, int on);
 int gimic_ctx_tile_stats(const gimic_ctx *ctx, double *out);
 
