@@ -132,3 +132,51 @@ def test_plane_grid_generator_rounding_matches_grid_points():
     w = (w3[k] * w2[j]) * w1[i]
     assert np.array_equal(r, g.points())
     assert np.array_equal(w, g.weights())
+
+
+def test_plane_grid_descriptors_match_each_grid():
+    """the vectorised descriptors equal, bit for bit, each grid's own
+    origin, axes, normal, centre and 1-D rules, layered grids included"""
+    from gimic_amd import grid as gridmod
+    from gimic_amd.lib import plane_grid_descriptors
+    coords = np.array([[0.0, 0.0, 0.0], [1.39, 0.21, -0.05], [2.1, 1.4, 0.3]])
+    grids, fields = [], []
+    for k, dist in enumerate(np.linspace(-1.5, 2.5, 6)):
+        kw = {"_arg": "bond", "type": "lobatto", "gauss_order": 7, "bond": [1, 2],
+              "fixcoord": [0.3, -0.7, 2.0], "distance": float(dist), "height": [-4.0, 4.5],
+              "width": [-2.2, 9.8], "grid_points": [14 + 7 * k, 21 + 7 * (k % 3), 0],
+              "radius": 3.0 + k}
+        g = gridmod.new_grid(kw, coords)
+        grids.append(g)
+        fields.append(gridmod.get_magnet(g, "X"))
+    # one layer each (the direct path), then a layered grid (planes per layer)
+    d = plane_grid_descriptors(grids, fields)
+    for gi, g in enumerate(grids):
+        assert np.array_equal(d["centers"][gi], g.center())
+        assert np.array_equal(d["normals"][gi], g.normal())
+        assert np.array_equal(d["axes"][gi], g.basv.T.reshape(-1))
+        assert tuple(d["p3w3"][gi]) == (g.pts[2][0], g.wgt[2][0])
+    assert np.array_equal(d["nodes2"], np.concatenate([g.pts[1] for g in grids]))
+    assert list(d["n1"]) == [len(g.pts[0]) for g in grids]
+    grids[3].pts[2] = np.array([-0.2, 0.0, 0.25])
+    grids[3].wgt[2] = np.array([0.3, 0.5, 0.2])
+    d = plane_grid_descriptors(grids, fields)
+    rows = [(gi, k) for gi, g in enumerate(grids) for k in range(len(g.pts[2]))]
+    assert len(d["n1"]) == len(rows) == 8
+    o1 = o2 = 0
+    for r, (gi, k) in enumerate(rows):
+        g = grids[gi]
+        assert d["owner"][r] == gi
+        assert d["n1"][r] == len(g.pts[0]) and d["n2"][r] == len(g.pts[1])
+        assert np.array_equal(d["origins"][r], g.origin)
+        assert np.array_equal(d["axes"][r], g.basv.T.reshape(-1))
+        assert np.array_equal(d["normals"][r], g.normal())
+        assert np.array_equal(d["centers"][r], g.center())
+        assert np.array_equal(d["bfields"][r], np.asarray(fields[gi], dtype=np.float64))
+        assert d["radii"][r] == g.radius
+        assert tuple(d["p3w3"][r]) == (g.pts[2][k], g.wgt[2][k])
+        assert np.array_equal(d["nodes1"][o1:o1 + d["n1"][r]], g.pts[0])
+        assert np.array_equal(d["wts2"][o2:o2 + d["n2"][r]], g.wgt[1])
+        o1 += d["n1"][r]
+        o2 += d["n2"][r]
+    assert o1 == len(d["nodes1"]) and o2 == len(d["wts2"])
