@@ -118,27 +118,34 @@ __global__ __launch_bounds__(256) void radix_hist_kernel(const unsigned *keys, i
     if (t < kBins) hist[t * nb + b] = cnt[t];
 }
 
-// exclusive scan of m counts in place (one workgroup)
-__global__ __launch_bounds__(256) void radix_scan_kernel(int *v, int m) {
-    __shared__ int s[256];
-    const int t = threadIdx.x;
-    const int per = (m + 255) / 256;
-    const int beg = min(m, t * per), end = min(m, beg + per);
-    int sum = 0;
-    for (int i = beg; i < end; i++) sum += v[i];
-    s[t] = sum;
-    __syncthreads();
-    for (int w = 1; w < 256; w <<= 1) {
-        const int x = t >= w ? s[t - w] : 0;
+// exclusive scan of m counts in place (one workgroup of 1024): the counts
+// are read in coalesced passes of 1024, each scanned within wavefronts
+// (shuffles), then across the 16 wavefronts, plus the running total
+__global__ __launch_bounds__(1024) void radix_scan_kernel(int *v, int m) {
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int carry = 0;
+    for (int base = 0; base < m; base += 1024) {
+        const int i = base + t;
+        const int x = i < m ? v[i] : 0;
+        int inc = x;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(inc, d);
+            if (lane >= d) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
         __syncthreads();
-        s[t] += x;
-        __syncthreads();
-    }
-    int run = s[t] - sum;
-    for (int i = beg; i < end; i++) {
-        const int x = v[i];
-        v[i] = run;
-        run += x;
+        int before = carry, total = carry;
+#pragma unroll
+        for (int w = 0; w < 16; w++) {
+            const int ws = wsum[w];
+            if (w < wave) before += ws;
+            total += ws;
+        }
+        if (i < m) v[i] = before + inc - x;
+        carry = total;
+        __syncthreads();  // wsum is rewritten by the next pass
     }
 }
 
@@ -332,7 +339,7 @@ hipError_t launch_sort_points(const double *pts, int n, double *spts, int *perm,
         int *iin = pass & 1 ? i1 : i0, *iout = pass & 1 ? i0 : i1;
         hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(256), 0, s, (const unsigned *)kin, n,
                            shift, hist);
-        hipLaunchKernelGGL(radix_scan_kernel, dim3(1), dim3(256), 0, s, hist, kBins * nb);
+        hipLaunchKernelGGL(radix_scan_kernel, dim3(1), dim3(1024), 0, s, hist, kBins * nb);
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(256), 0, s, (const unsigned *)kin,
                            (const int *)iin, n, shift, (const int *)hist, kout, iout);
     }
