@@ -778,6 +778,40 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             }
             hip_check(launch_basis_eval(ea, eval_mode(variant), st), "basis_eval");
         };
+        // tensor assembly (and flux contributions) of this chunk from the
+        // partial sums of the last contraction launch
+        auto finalize = [&](int variant) {
+            FinalizeArgs fa{};
+            fa.part = c->part.as<double>();
+            fa.nsplit = ca.nsplit;
+            fa.part_rows = presplit(variant) ? 1 : 0;
+            fa.rows = rows;
+            fa.npts = n_c;
+            fa.nmatset = nmatset;
+            fa.pts = spts;
+            fa.perm = c->perm.as<int>();
+            fa.giao = c->giao;
+            fa.diamag = c->diamag;
+            fa.paramag = c->paramag;
+            // outputs: this chunk's slice, or (subset run) the whole arrays
+            // addressed through out_idx
+            const size_t o0 = out_idx ? 0 : start;
+            fa.pidx = out_idx ? out_idx + start : nullptr;
+            fa.jt = d_jt + 9 * o0;
+            fa.jt_stride = 9 * (long long)(out_idx ? out_npts : npts);
+            if (flux) {
+                fa.w = flux->w + o0;
+                fa.plane = flux->plane ? flux->plane + o0 : nullptr;
+                fa.pnorm = flux->pnorm;
+                fa.pbfield = flux->pbfield;
+                fa.pcenter = flux->pcenter;
+                fa.pradius = flux->pradius;
+                fa.contrib = flux->contrib + o0;
+                fa.contrib_mod = flux->contrib_mod ? flux->contrib_mod + o0 : nullptr;
+                fa.contrib_acid = flux->contrib_acid ? flux->contrib_acid + o0 : nullptr;
+            }
+            hip_check(launch_finalize(fa, st), "finalize");
+        };
         const bool tune = prec == GIMIC_PREC_MIXED && c->mixed_variant < 0;
         if (capture && tune) throw Error(GIMIC_ESTATE, "mixed kernel not tuned before capture");
         if (mode == EvalMode::Reserve && !tune) {
@@ -793,7 +827,8 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         });
         if (tune) {
             // autotune once per context on this chunk: basis evaluation +
-            // contraction for the pre-split kernels that walk block lists,
+            // contraction + tensor assembly (which reads one partial sum per
+            // N-split) for the pre-split kernels that walk block lists,
             // each with several N-split counts (more splits give more
             // workgroups that share each row tile's A through L2, fewer
             // give each workgroup more N-blocks to amortise its start);
@@ -815,6 +850,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
                     hip_check(hipEventRecord(e0, st), "hipEventRecord");
                     eval(v);
                     run(v, sp);
+                    finalize(v);
                     hip_check(hipEventRecord(e1, st), "hipEventRecord");
                     hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
                     float t = 0.f;
@@ -840,38 +876,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         const int variant = prec == GIMIC_PREC_FP64 ? 0 : c->mixed_variant;
         timed(c, 0, st, [&] { eval(variant); });
         timed(c, 1, st, [&] { run(variant); });
-        const int nsplit = ca.nsplit;
-
-        FinalizeArgs fa{};
-        fa.part = c->part.as<double>();
-        fa.nsplit = nsplit;
-        fa.part_rows = presplit(variant) ? 1 : 0;
-        fa.rows = rows;
-        fa.npts = n_c;
-        fa.nmatset = nmatset;
-        fa.pts = spts;
-        fa.perm = c->perm.as<int>();
-        fa.giao = c->giao;
-        fa.diamag = c->diamag;
-        fa.paramag = c->paramag;
-        // outputs: this chunk's slice, or (subset run) the whole arrays
-        // addressed through out_idx
-        const size_t o0 = out_idx ? 0 : start;
-        fa.pidx = out_idx ? out_idx + start : nullptr;
-        fa.jt = d_jt + 9 * o0;
-        fa.jt_stride = 9 * (long long)(out_idx ? out_npts : npts);
-        if (flux) {
-            fa.w = flux->w + o0;
-            fa.plane = flux->plane ? flux->plane + o0 : nullptr;
-            fa.pnorm = flux->pnorm;
-            fa.pbfield = flux->pbfield;
-            fa.pcenter = flux->pcenter;
-            fa.pradius = flux->pradius;
-            fa.contrib = flux->contrib + o0;
-            fa.contrib_mod = flux->contrib_mod ? flux->contrib_mod + o0 : nullptr;
-            fa.contrib_acid = flux->contrib_acid ? flux->contrib_acid + o0 : nullptr;
-        }
-        timed(c, 2, st, [&] { hip_check(launch_finalize(fa, st), "finalize"); });
+        timed(c, 2, st, [&] { finalize(variant); });
     }
     if (prec == GIMIC_PREC_MIXED && mode == EvalMode::Eager && !out_idx)
         refine_near_nuclei(c, npts, d_r, d_jt, spins, st, flux);
