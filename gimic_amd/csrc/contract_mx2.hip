@@ -114,6 +114,30 @@ __device__ __forceinline__ void stage_block_k8(const __bf16 *Bnt, int4 ent, cons
         if (i < np) stage_piece_k8<WAVES>(Bnt, ent, bzero, buf, wave, lane, i);
 }
 
+// One step of a transposing sum over the 16 lanes of a DPP row: lanes whose
+// `up` bit is set keep the upper W of the values they still hold, the others
+// the lower W, and each adds the copy its partner (DPP control CTRL, a lane
+// that differs in that bit and holds the same values) does not keep.
+template <int W, int CTRL>
+__device__ __forceinline__ void xsum_step(float (&v)[32], bool up) {
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const float lo = v[i], hi = v[i + W];
+        const float send = up ? lo : hi, keep = up ? hi : lo;
+        v[i] = keep + GIMIC_DPP(send, CTRL);
+    }
+}
+
+// v[0..31] summed over the 16 lanes of each DPP row, four exchanges of
+// 16, 8, 4 and 2 values (row mirror, half-row mirror, quad xor 2, quad xor
+// 1): lane c is left with the sums of slots 2c and 2c + 1 in v[0], v[1]
+__device__ __forceinline__ void row16_xsum(float (&v)[32], int c) {
+    xsum_step<16, 0x140>(v, (c & 8) != 0);
+    xsum_step<8, 0x141>(v, (c & 4) != 0);
+    xsum_step<4, 0x4E>(v, (c & 2) != 0);
+    xsum_step<2, 0xB1>(v, (c & 1) != 0);
+}
+
 __device__ __forceinline__ v4f mfma6(const v8bf (&a)[3], const v8bf (&b)[3], v4f acc) {
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
@@ -364,14 +388,16 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
 #pragma unroll
                         for (int m = 0; m < 3; m++) q[13 + m + 3 * d] += G[m] * acc[rb][2 * (4 + d) + h][r];
                 }
+                // slots 2k, 2k + 1 = sums k, k + 16: lane c ends with the row
+                // sums c and c + 16, the two it adds to the partial buffer
+                float v[32];
 #pragma unroll
-                for (int i = 0; i < kNQ; i++) q[i] = row16_sum(q[i]);
-                float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-                for (int i = 0; i < kNQ; i++) {
-                    if (i < 16) s0 = i == c ? q[i] : s0;
-                    else s1 = i - 16 == c ? q[i] : s1;
+                for (int k = 0; k < 16; k++) {
+                    v[2 * k] = q[k];
+                    v[2 * k + 1] = k + 16 < kNQ ? q[k + 16] : 0.f;
                 }
+                row16_xsum(v, c);
+                const float s0 = v[0], s1 = v[1];
                 double *o = pout + (row0 + rl) * kNQ;
                 unsafeAtomicAdd(o + c, (double)s0);
                 if (c + 16 < kNQ) unsafeAtomicAdd(o + c + 16, (double)s1);
