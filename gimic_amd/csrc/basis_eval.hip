@@ -30,6 +30,18 @@ __constant__ double c_sph[466];  // T_l, l = 2..5, (2l+1) x ncart(l) row-major
 // groups of four before any exponential, so a group costs one memory
 // latency rather than one per primitive (the terms are summed in the same
 // order as a plain loop).
+// e^x to FP32 accuracy (relative error ~1e-7) for the mixed path, whose
+// basis values are rounded to FP32 anyway: x log2(e) = n + f in FP64 with n
+// integer and |f| <= 1/2, 2^f from the hardware exp2 (v_exp_f32), scaled by
+// 2^n in FP64 (no FP32 overflow or underflow for any n)
+__device__ __forceinline__ double exp_f32acc(double x) {
+    const double t = x * 1.4426950408889634;  // log2(e)
+    const double n = rint(t);
+    const float p = __builtin_amdgcn_exp2f((float)(t - n));
+    return ldexp((double)p, (int)fmax(n, -2000.0));
+}
+
+template <bool FAST = false>
 __device__ __forceinline__ void radial_parts(const EvalArgs &a, const DevShell &sh, double s,
                                              double &g, double &h) {
     g = 0.0;
@@ -45,7 +57,7 @@ __device__ __forceinline__ void radial_parts(const EvalArgs &a, const DevShell &
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (p0 + k < sh.nprim) {
-                const double q = cc[k] * exp(-xp[k] * s);
+                const double q = cc[k] * (FAST ? exp_f32acc(-xp[k] * s) : exp(-xp[k] * s));
                 g += q;
                 h += xp[k] * q;
             }
@@ -347,7 +359,7 @@ void basis_eval_staged_kernel(EvalArgs a) {
             const double s = rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
             if (sqrt(s) > sh.radius) continue;  // screened: zeros already in place
             double g, h;
-            radial_parts(a, sh, s, g, h);
+            radial_parts<!F64>(a, sh, s, g, h);
             constexpr int M = F64 ? 4 : 3;
             if (a.turbomole) {
                 if (a.spherical) dispatch_l<M, true, true, LMAX>(b, lp, sh, rr, g, h);
