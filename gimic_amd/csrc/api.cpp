@@ -628,7 +628,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
     grow(c->toff, (rows_cap / 128 + 1) * 4, false);
     grow(c->k8count, rows_cap / 128 * 4, false);
     grow(c->k8list, rows_cap / 128 * nblk * 4 * 4, false);
-    grow(c->bzero, 12 * 1024, true);  // stays zero: never written
+    grow(c->bzero, 12 * 1024, c->bzero.bytes < 12 * 1024);  // zeroed once, never written
     const int nmatset = M.nmatset;
     for (size_t start = 0; start < (size_t)npts; start += chunk) {
         const int n_c = (int)std::min(chunk, (size_t)npts - start);
