@@ -34,7 +34,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"mixed": 2500.0, "fp64": 78.6}  # MI355X dense MFMA (bf16, fp64)
+PEAK_TFLOPS = {"mixed": 2500.0, "fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA (bf16, fp64, fp32)
 HBM_GBS = 8000.0
 
 
@@ -482,6 +482,9 @@ def main():
     # (FP32 operands split into three BF16 parts): matrix-pipe use is 6x frac
     roof["mfma_issued_tflops"] = round(6.0 * roof["achieved"], 1)
     roof["mfma_issued_frac"] = round(6.0 * roof["achieved"] / roof["peak"], 4)
+    # the arithmetic emulated is FP32 (products of FP32 operands, FP32
+    # accumulation): against MI355X's dense FP32 matrix peak
+    roof["over_fp32_mfma_peak"] = round(roof["achieved"] / PEAK_TFLOPS["fp32"], 3)
     roof["traffic"] = measured_traffic(roof["kernel"], split, roof["points_per_launch"])
     if roof["traffic"] is not None:
         roof["traffic_note"] = ("L2-miss bytes per launch (Infinity Cache + HBM), rocprofv3 FETCH_SIZE/"
