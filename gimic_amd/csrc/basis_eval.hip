@@ -344,10 +344,12 @@ void basis_eval_staged_kernel(EvalArgs a) {
         __syncthreads();
         const int items = nr * a.nshell;
         for (int it = tid; it < items; it += 256) {
-            const int lp = it / a.nshell;
+            // shell-major: the nr rows of one shell are neighbouring lanes,
+            // which load the same shell and primitive data
+            const int lp = it % nr;
             const size_t row = row0 + r0 + lp;
             if (row >= (size_t)a.npts) continue;  // padding rows stay zero
-            DevShell sh = a.shells[it % a.nshell];
+            DevShell sh = a.shells[it / nr];
             if (bpos) {
                 const int cb = bpos[sh.fout >> 5];
                 if (cb < 0) continue;  // zero over the whole tile
