@@ -827,8 +827,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         });
         if (tune) {
             // autotune once per context on this chunk: basis evaluation +
-            // contraction + tensor assembly (which reads one partial sum per
-            // N-split) for the pre-split kernels that walk block lists,
+            // contraction for the pre-split kernels that walk block lists,
             // each with several N-split counts (more splits give more
             // workgroups that share each row tile's A through L2, fewer
             // give each workgroup more N-blocks to amortise its start);
@@ -850,7 +849,6 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
                     hip_check(hipEventRecord(e0, st), "hipEventRecord");
                     eval(v);
                     run(v, sp);
-                    finalize(v);
                     hip_check(hipEventRecord(e1, st), "hipEventRecord");
                     hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
                     float t = 0.f;
