@@ -99,9 +99,9 @@ def kernel_roofline(ctx, d_pts, out, precision, nbf):
     npts = d_pts.shape[0]
     dense = 14.0 * nbf * nbf * npts
     if ts["tiles"] > 0:
-        # the pre-split mixed kernels step K through live 8-function groups,
-        # the FP64 kernel through whole listed blocks
-        kn = ts["blocks_kn"] if precision != "fp64" else ts["blocks_sq"]
+        # both the pre-split mixed kernels and the FP64 kernel step K through
+        # the live 8-function groups of the listed blocks
+        kn = ts["blocks_kn"]
         flop = 14.0 * ts["tile_rows"] * 1024.0 * kn
         block_pairs = kn / (ts["tiles"] * ts["nblk"] ** 2)
     else:  # a dense kernel

@@ -18,13 +18,14 @@
 // Tiling: 256 threads = 4 wavefronts; a workgroup owns 128 point rows
 // (32 per wavefront: two 16-row MFMA blocks) and walks a range of 16-column
 // N-tiles; per N-tile each wavefront keeps 2 x 7 v_mfma_f64_16x16x4 blocks.
-// Only the 32-function blocks listed for the row tile (those that screening
-// does not zero over it, tiles.hip) are visited, along K and along N: the
-// N-tiles are the listed blocks' halves, split evenly over the workgroups
-// that share the row tile.
-// The K index is remapped inside every 16-wide K-step so each lane loads 32
+// Only what screening leaves non-zero over the row tile (tiles.hip) is
+// visited: along N the listed 32-function blocks' 16-column halves, split
+// evenly over the workgroups that share the row tile; along K the live
+// 8-function groups of the listed blocks, two per 16-wide K-step.
+// The K index is remapped inside every K-step so each lane loads 32
 // contiguous bytes of its A row and of each B column (lane group g holds
-// k = 4g..4g+3; sub-step s feeds k = 4g+s to both operands).
+// elements 4(g&1)..+3 of group g/2; sub-step s feeds the s-th of them to
+// both operands).
 #include "device_common.h"
 
 namespace gimic {
@@ -44,34 +45,44 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs a) {
     for (int i = tid; i < kWgRows * kNQ; i += 256) qs[i] = 0.0;
     __syncthreads();
 
-    const double *A0 = a.a64 + (row0 + wave * 32 + c) * lda + 4 * g;
+    const double *A0 = a.a64 + (row0 + wave * 32 + c) * lda;
     const double *A1 = A0 + 16 * (size_t)lda;
     const double *Bset = a.b64 + (size_t)set * kNMat * matsz;
     const int e0 = (int)((long long)split * nk / a.nsplit);
     const int e1 = (int)((long long)(split + 1) * nk / a.nsplit);
-    // first element of listed K-step kk
+    // first element of listed N-tile e (the listed blocks' 16-column halves)
     auto kof = [&](int kk) { return L[kk >> 1] * 32 + (kk & 1) * 16; };
+    // K walks the live 8-function groups of the listed blocks, two per
+    // 16-wide K-step (tiles.hip): lane group g takes group 2 kk + g / 2,
+    // elements 4 (g & 1) .. + 3.  A padding entry (-1) reads A from the
+    // zero buffer (B from any valid place: the product is zero).
+    const int *K8 = a.k8list + (size_t)blockIdx.x * a.nblk * 4;
+    const int nks = (a.k8count[blockIdx.x] + 1) >> 1;
+    const double *zero = (const double *)a.bzero;
+    auto kent = [&](int kk) { return K8[2 * kk + (g >> 1)]; };
+    auto koff = [&](int v) { return v < 0 ? 0 : (v >> 12) * 8 + (g & 1) * 4; };
 
     for (int e = e0; e < e1; e++) {
         const int nt = kof(e) / kNT;
         const int col = nt * kNT + c;
-        const double *Bc = Bset + (size_t)col * lda + 4 * g;
+        const double *Bc = Bset + (size_t)col * lda;
         v4d acc[2][kNMat];
 #pragma unroll
         for (int m = 0; m < kNMat; m++) acc[0][m] = acc[1][m] = v4d{0.0, 0.0, 0.0, 0.0};
 
-        const int k0 = kof(0);
-        v4d a0 = *(const v4d *)(A0 + k0), a1 = *(const v4d *)(A1 + k0);
+        const int v0 = kent(0), k0 = koff(v0);
+        v4d a0 = *(const v4d *)(v0 < 0 ? zero : A0 + k0), a1 = *(const v4d *)(v0 < 0 ? zero : A1 + k0);
         v4d b[kNMat];
 #pragma unroll
         for (int m = 0; m < kNMat; m++) b[m] = *(const v4d *)(Bc + m * matsz + k0);
-        // element offset of the next listed K-step, read one step early
-        int knext = kof(nk > 1 ? 1 : 0);
-        for (int kk = 0; kk < nk; kk++) {
+        // group entry of the next K-step, read one step early
+        int vnext = kent(nks > 1 ? 1 : 0);
+        for (int kk = 0; kk < nks; kk++) {
             // prefetch the next K-step while this one runs on the matrix cores
-            const int kn = knext;
-            knext = kof(kk + 2 < nk ? kk + 2 : nk - 1);
-            const v4d na0 = *(const v4d *)(A0 + kn), na1 = *(const v4d *)(A1 + kn);
+            const int vn = vnext, kn = koff(vn);
+            vnext = kent(kk + 2 < nks ? kk + 2 : nks - 1);
+            const v4d na0 = *(const v4d *)(vn < 0 ? zero : A0 + kn);
+            const v4d na1 = *(const v4d *)(vn < 0 ? zero : A1 + kn);
             v4d nb[kNMat];
 #pragma unroll
             for (int m = 0; m < kNMat; m++) nb[m] = *(const v4d *)(Bc + m * matsz + kn);

@@ -168,10 +168,10 @@ int gimic_ctx_fp64_refined(const gimic_ctx *ctx, double *radius, int64_t *npts);
  * up.  out[6]: row tiles, sum of listed blocks, sum of squared listed
  * blocks, rows per tile, blocks per dense row (lda / 32), and the sum over
  * tiles of listed blocks x live 8-function groups / 4.  A tile of R rows
- * with n listed blocks costs 2 * R * 7 * (32 n)^2 flops per density set
- * when its K dimension walks whole blocks (the FP64 kernel), and
- * 2 * R * 7 * 32 n * 8 k with k live groups (the pre-split mixed kernels,
- * whose K-steps take four live groups). */
+ * with n listed blocks and k live groups costs 2 * R * 7 * 32 n * 8 k
+ * flops per density set (the FP64 and pre-split mixed kernels step K
+ * through the live groups; a kernel that walked whole blocks would do
+ * 2 * R * 7 * (32 n)^2). */
 int gimic_ctx_set_tile_stats(gimic_ctx *ctx, int on);
 int gimic_ctx_tile_stats(const gimic_ctx *ctx, double *out);
 
