@@ -630,7 +630,13 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
     grow(c->k8list, rows_cap / 128 * nblk * 4 * 4, false);
     grow(c->bzero, 12 * 1024, c->bzero.bytes < 12 * 1024);  // zeroed once, never written
     const int nmatset = M.nmatset;
-    for (size_t start = 0; start < (size_t)npts; start += chunk) {
+    // chunks in rotated order, the middle one first: the autotuner times the
+    // first chunk it sees, and the middle of a grid around a molecule is
+    // where the work is densest (each chunk's results are its own, so the
+    // order changes nothing else)
+    const size_t nchunks = ((size_t)npts + chunk - 1) / chunk;
+    for (size_t ci = 0; ci < nchunks; ci++) {
+        const size_t start = ((ci + nchunks / 2) % nchunks) * chunk;
         const int n_c = (int)std::min(chunk, (size_t)npts - start);
         const int rows = (int)round_up((size_t)n_c, gran);
 

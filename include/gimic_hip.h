@@ -133,7 +133,7 @@ int gimic_ctx_get_timing(gimic_ctx *ctx, double *ms, long long *launches);
  * buffered.  Kernels 3 to 5, 8 and 9 (and the FP64 kernel) visit only the
  * 32-function blocks that screening leaves non-zero over their row tile;
  * 1, 2, 6 and 7 compute the dense product.  The first mixed evaluation of a
- * context times basis evaluation + contraction on its first chunk for
+ * context times basis evaluation + contraction on its middle chunk for
  * kernels 3, 4 and 5 (N-split count 4) and 8 and 9 (N-split counts 2, 4
  * and 8), each twice, and keeps the fastest.  Returns the choice (variant
  * -1 before the first mixed call), its split count (0: chosen by workgroup
