@@ -55,10 +55,12 @@
 // hardware hands workgroup i to XCD i mod 8, so slot s = i / 8 of XCD x runs
 // the (s / nsplit)-th row tile of that XCD with split s % nsplit, and they
 // read the same A rows at the same time, all but one from the XCD's L2; and
-// (2) an XCD takes runs of kTileRun consecutive row tiles (runs go to the
-// XCDs in turn), so the tiles it runs together are neighbours along the
-// points' Morton order with nearly the same block lists, and their equal
-// splits stream the same B blocks through its L2 as well.
+// (2) the row tiles go to the XCDs in turn (runs of kTileRun = 1), so an
+// XCD runs tiles 8 apart along the points' Morton order: close enough that
+// their lists mostly agree, and spread enough that its workgroups do not
+// all stream the same B groups at once.  On the 60-atom cage, runs of 2, 4
+// and 32 tiles were 0.3%, 1.3% and 12% slower; runs of 8 were 1-5% slower
+// (two boxes).
 #include "device_common.h"
 
 namespace gimic {
@@ -152,7 +154,7 @@ __device__ __forceinline__ v4f mfma6(const v8bf (&a)[3], const v8bf (&b)[3], v4f
 
 // Row tiles per XCD: runs of kTileRun consecutive tiles go to the XCDs in
 // turn, enough runs to cover every tile
-constexpr int kTileRun = 8;
+constexpr int kTileRun = 1;
 __host__ __device__ constexpr int tile_groups(int ntiles) {
     return (ntiles + 8 * kTileRun - 1) / (8 * kTileRun) * kTileRun;
 }
