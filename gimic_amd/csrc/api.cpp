@@ -89,6 +89,13 @@ struct DevBuf {
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
+// Device memory for the basis values and gradients of one chunk of points
+// (one pass of basis evaluation, contraction and assembly): a third of the
+// device's memory, at most kChunkBytesMax.  The total, not the free memory,
+// sets it, so the chunks (and with them the results' last bits, which the
+// tiles' block lists touch) are the same on every run on one device model.
+constexpr size_t kChunkBytesMax = (size_t)96 << 30;
+
 // Expected cost of a function order: the mean over sample points of
 // (number of live 32-function blocks)^2, the live (K-block, N-block) pairs
 // of a row tile at the point.  live[p * nsh + s]: shell s reaches sample p.
@@ -205,6 +212,7 @@ std::vector<int> choose_function_order(const Basis &b, const std::vector<int> &a
 struct gimic_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    size_t chunk_bytes = 0;  // basis buffers of one chunk of points
     Basis basis;
     double screen_thrs = 1e-6;
     int giao = 1, diamag = 1, paramag = 1;
@@ -290,6 +298,9 @@ namespace {
 void setup_device(gimic_ctx *c) {
     hip_check(hipGetDevice(&c->device), "hipGetDevice");
     hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
+    size_t mem_free = 0, mem_total = 0;
+    hip_check(hipMemGetInfo(&mem_free, &mem_total), "hipMemGetInfo");
+    c->chunk_bytes = std::max<size_t>((size_t)1 << 30, std::min(kChunkBytesMax, mem_total / 3));
     Basis &b = c->basis;
     finalize_basis(b, c->screen_thrs);
     c->lda = (int)round_up((size_t)b.nbf, kKPad);
@@ -595,15 +606,18 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
 
     const size_t ld = (size_t)c->lda;
     // chunk of points per pass: basis values + gradients of one chunk stay
-    // within ~24 GiB (FP64: 32 bytes per point and function)
+    // within c->chunk_bytes (FP64: 32 bytes per point and function)
     // (mixed: FP32 values 4 + split values 6 + FP32 gradients 12 bytes)
     const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 22);
     // row granularity: a multiple of every contraction kernel's workgroup
     // height (128; 256 for the 256-row pre-split kernel)
     const size_t gran = prec == GIMIC_PREC_FP64 ? (size_t)kWgRows : 256;
-    size_t cap = (size_t)24 << 30;
-    size_t chunk = std::max<size_t>(gran, (cap / per_row) / gran * gran);
-    chunk = std::min(chunk, round_up((size_t)npts, gran));
+    const size_t rows_all = round_up((size_t)npts, gran);
+    size_t chunk = std::max<size_t>(gran, (c->chunk_bytes / per_row) / gran * gran);
+    // as few chunks as the cap allows, all of one size (a short last chunk
+    // would be a launch of its own with a tail that fills the GPU badly)
+    const size_t nchunk_min = (rows_all + chunk - 1) / chunk;
+    chunk = round_up((rows_all + nchunk_min - 1) / nchunk_min, gran);
     const size_t rows_cap = chunk;
     const long long gplane = (long long)(rows_cap * ld);
     // padding columns and rows of the basis buffers must read as zero: they
