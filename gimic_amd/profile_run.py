@@ -6,9 +6,10 @@ Runs, on cuda:0, (1) a device-to-device copy of exactly 1 GiB (torch), the
 reference for scaling FETCH_SIZE / WRITE_SIZE counters to bytes, then (2) R
 J-tensor evaluations of the first N points of the 128^3 grid around the
 60-atom cage (nbf 1800) in the chosen precision (mixed without the FP64
-pass over points next to a nucleus).  With N at most one chunk
-(642,048 points mixed, 441,472 FP64) every evaluation is one launch each of
-the sort, basis evaluation, contraction and tensor assembly kernels.
+pass over points next to a nucleus).  With N at most one chunk (on a
+288 GB MI355X, the whole grid mixed and 1,048,576 points FP64) every
+evaluation is one launch each of the sort, basis evaluation, contraction and
+tensor assembly kernels.
 """
 import argparse
 import time
