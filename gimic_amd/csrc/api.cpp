@@ -737,13 +737,15 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
                 c->tile_acc[3] = tile_rows;
             }
         };
-        // work split over N so a launch has at least ~2048 workgroups, or
-        // the split count the tuner chose for the pre-split kernel
+        // work split over N so a launch has at least ~8192 workgroups (32
+        // per CU; small FP64 launches, such as the pass over the points next
+        // to nuclei, ran 4.5% faster than at 2048 and 22% faster than at
+        // 512), or the split count the tuner chose for the pre-split kernel
         auto plan = [&](int wg_rows, int tile_cols, int want_split = 0) {
             ca.ntiles = c->ncol_pad / tile_cols;
             const long long per = (long long)(rows / wg_rows) * nmatset;
             int nsplit = (int)std::min<long long>(ca.ntiles,
-                                                  std::max<long long>(1, (2048 + per - 1) / per));
+                                                  std::max<long long>(1, (8192 + per - 1) / per));
             if (want_split > 0) nsplit = std::min(want_split, ca.ntiles);
             ca.tiles_per_split = (ca.ntiles + nsplit - 1) / nsplit;
             ca.nsplit = (ca.ntiles + ca.tiles_per_split - 1) / ca.tiles_per_split;
