@@ -213,6 +213,7 @@ struct gimic_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     size_t chunk_bytes = 0;  // basis buffers of one chunk of points
+    size_t chunk_bytes_default = 0;
     Basis basis;
     double screen_thrs = 1e-6;
     int giao = 1, diamag = 1, paramag = 1;
@@ -300,7 +301,9 @@ void setup_device(gimic_ctx *c) {
     hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
     size_t mem_free = 0, mem_total = 0;
     hip_check(hipMemGetInfo(&mem_free, &mem_total), "hipMemGetInfo");
-    c->chunk_bytes = std::max<size_t>((size_t)1 << 30, std::min(kChunkBytesMax, mem_total / 3));
+    c->chunk_bytes_default =
+        std::max<size_t>((size_t)1 << 30, std::min(kChunkBytesMax, mem_total / 3));
+    c->chunk_bytes = c->chunk_bytes_default;
     Basis &b = c->basis;
     finalize_basis(b, c->screen_thrs);
     c->lda = (int)round_up((size_t)b.nbf, kKPad);
@@ -1198,7 +1201,8 @@ void flux_run(gimic_ctx *c, int64_t npts, int nplanes, const int64_t *offs, cons
             (uintptr_t)c->mixed_split, (uintptr_t)c->spts.p, (uintptr_t)c->perm.p,
             (uintptr_t)c->bpart.p, (uintptr_t)c->sortscr.p, (uintptr_t)c->bcount.p,
             (uintptr_t)c->blist.p, (uintptr_t)c->toff.p, (uintptr_t)c->k8count.p,
-            (uintptr_t)c->k8list.p, (uintptr_t)c->bzero.p, (uintptr_t)refine};
+            (uintptr_t)c->k8list.p, (uintptr_t)c->bzero.p, (uintptr_t)refine,
+            (uintptr_t)c->chunk_bytes};
         key.insert(key.end(), source.begin(), source.end());
         if (!c->graph_exec || key != c->graph_key) {
             if (c->graph_exec) {
@@ -1386,6 +1390,21 @@ int gimic_ctx_set_fp64_radius(gimic_ctx *c, double radius) {
         if (!(radius >= 0.0) || !std::isfinite(radius))
             throw Error(GIMIC_EINVAL, "radius must be finite and >= 0");
         c->fp64_radius = radius;
+    });
+}
+
+int gimic_ctx_set_chunk_bytes(gimic_ctx *c, int64_t bytes) {
+    return guarded([&] {
+        if (!c) throw Error(GIMIC_EINVAL, "null context");
+        if (bytes < 0) throw Error(GIMIC_EINVAL, "chunk bytes must be >= 0");
+        c->chunk_bytes = bytes == 0 ? c->chunk_bytes_default : (size_t)bytes;
+    });
+}
+
+int gimic_ctx_chunk_bytes(const gimic_ctx *c, int64_t *bytes) {
+    return guarded([&] {
+        if (!c || !bytes) throw Error(GIMIC_EINVAL, "null argument");
+        *bytes = (int64_t)c->chunk_bytes;
     });
 }
 

@@ -19,7 +19,7 @@ EXPORTS = [
     "gimic_set_screening", "gimic_calc_jtensor", "gimic_calc_jvector", "gimic_calc_modj",
     "gimic_get_gauss_points", "mkgausspoints", "gimic_last_error", "gimic_ctx_create_files",
     "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
-    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_tuning_log", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_tile_stats", "gimic_ctx_tile_stats", "gimic_ctx_set_fp64_radius", "gimic_ctx_fp64_refined", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
+    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_tuning_log", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_tile_stats", "gimic_ctx_tile_stats", "gimic_ctx_set_fp64_radius", "gimic_ctx_fp64_refined", "gimic_ctx_set_chunk_bytes", "gimic_ctx_chunk_bytes", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
     "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_flux_plane_grids", "gimic_ctx_properties", "gimic_sph_transform", "gimic_quadrature",
     "gimic_selftest_mfma_layout", "gimic_selftest_mfma32_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_set_stream", "gimic_mlp_info",
     "gimic_mlp_params", "gimic_mlp_loss_grad", "gimic_mlp_train_step", "gimic_mlp_predict",
@@ -80,6 +80,8 @@ def load():
         "gimic_ctx_set_tile_stats": [P, I],
         "gimic_ctx_set_fp64_radius": [P, D],
         "gimic_ctx_fp64_refined": [P, P, P],
+        "gimic_ctx_set_chunk_bytes": [P, I64],
+        "gimic_ctx_chunk_bytes": [P, P],
         "gimic_ctx_tile_stats": [P, P],
         "gimic_ctx_set_graph": [P, I],
         "gimic_ctx_graph_stats": [P, P, P],
@@ -294,6 +296,17 @@ class Context:
         """mixed precision: points within `radius` bohr of a nucleus are
         evaluated again in FP64 (default 0.4; 0 turns it off)"""
         check(self._lib.gimic_ctx_set_fp64_radius(self._h, float(radius)))
+
+    def set_chunk_bytes(self, nbytes):
+        """device memory for the basis buffers of one chunk of points (0: the
+        default, a third of the device's memory and at most 96 GiB)"""
+        check(self._lib.gimic_ctx_set_chunk_bytes(self._h, int(nbytes)))
+
+    def chunk_bytes(self):
+        """the chunk memory in force, in bytes"""
+        n = ctypes.c_int64(0)
+        check(self._lib.gimic_ctx_chunk_bytes(self._h, ctypes.byref(n)))
+        return n.value
 
     def fp64_refined(self):
         """(radius, points the last mixed call evaluated in FP64)"""

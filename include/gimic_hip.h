@@ -161,6 +161,21 @@ int gimic_ctx_set_mixed_kernel(gimic_ctx *ctx, int variant, int split);
 int gimic_ctx_set_fp64_radius(gimic_ctx *ctx, double radius);
 int gimic_ctx_fp64_refined(const gimic_ctx *ctx, double *radius, int64_t *npts);
 
+/* Device memory for the basis values and gradients of one chunk of points
+ * (no counterpart in the reference, which evaluates point by point).  A call
+ * evaluates its points in chunks of equal size whose basis buffers fit in
+ * `bytes` (mixed: 22 bytes per point and padded function; FP64: 32), and at
+ * least 256 points; 0 restores the default, a third of the device's total
+ * memory and at most 96 GiB (one launch for a 128^3 grid with 1800
+ * functions).  Each launch ends in a tail that leaves the GPU partly idle,
+ * so fewer, larger chunks are faster.  Buffers a context has allocated are
+ * kept until it is destroyed: set this before the first evaluation to bound
+ * its memory.  Results of different chunk sizes agree to the rounding of
+ * the precision (the tiles' block lists depend on the chunk's points).
+ * gimic_ctx_chunk_bytes reports the setting in force. */
+int gimic_ctx_set_chunk_bytes(gimic_ctx *ctx, int64_t bytes);
+int gimic_ctx_chunk_bytes(const gimic_ctx *ctx, int64_t *bytes);
+
 /* Screening block-list statistics (no counterpart in the reference).  While
  * enabled (on != 0; enabling clears the totals), every eager evaluation
  * that runs a block-list kernel copies its row tiles' block counts to the

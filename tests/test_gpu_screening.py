@@ -48,6 +48,32 @@ def test_result_does_not_depend_on_point_order(cage, precision):
     assert np.array_equal(ctx.jtensor(pts, precision=precision), J)
 
 
+def test_chunks_change_launches_not_results(cage):
+    """a chunk limit that splits the points into four or five launches gives
+    the tensors of one launch, to the rounding of each precision; 0 restores
+    the default and a negative size is refused"""
+    s, ctx = cage
+    pts = _grid(24, 0.15, [3.0, -1.8, -1.8])
+    default = ctx.chunk_bytes()
+    assert default >= 1 << 30
+    print("case cage60 chunk size", flush=True)
+    ref = {p: ctx.jtensor(pts, precision=p) for p in ("fp64", "mixed")}
+    small = 1824 * 22 * 4096  # 4096 mixed rows of 1824 padded functions
+    try:
+        ctx.set_chunk_bytes(small)
+        assert ctx.chunk_bytes() == small
+        j64 = ctx.jtensor(pts, precision="fp64")
+        jm = ctx.jtensor(pts, precision="mixed")
+    finally:
+        ctx.set_chunk_bytes(0)
+    assert ctx.chunk_bytes() == default
+    assert rel_err(j64, ref["fp64"]).max() <= 1e-12
+    assert rel_err(jm, ref["mixed"]).max() <= 1e-5
+    assert rel_err(jm, ref["fp64"]).max() <= 1e-5
+    with pytest.raises(RuntimeError):
+        ctx.set_chunk_bytes(-1)
+
+
 def test_lists_leave_out_blocks_and_fp64_matches_oracle(cage):
     s, ctx = cage
     pts = _grid(16, 0.2, [4.5, -1.5, -1.5])  # 4096 points near one side of the cage
