@@ -408,7 +408,7 @@ def bench_surrogate(args):
         per_tol[f"{tol:g}"] = {"served_fraction": round(float(served.float().mean()), 4),
                                "points_per_s": round(n / t_h, 1),
                                "gain_over_exact": round(t_ex / t_h, 3),
-                               "judge": sg.judge(grid, J, served)}
+                               "audit": sg.audit(grid, J, served)}
     ctx.close()
     head = per_tol["0.001"]
     return {

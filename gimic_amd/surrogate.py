@@ -11,7 +11,7 @@ stays in charge:
   out and a cell is served by the network only when its held-out relative
   error sqrt(sum |J_net - J|^2 / sum |J|^2) is at most `tol` and it holds
   at least `min_holdout` held-out points; every other cell is exact;
-* `judge()` evaluates a sample of the served points with the exact kernel and
+* `audit()` evaluates a sample of the served points with the exact kernel and
   reports the error actually delivered.
 
 The surrogate has no counterpart in the reference; its input is the same
@@ -268,7 +268,7 @@ class Surrogate:
             out[ex] = self.ctx.jtensor_device(r[ex], spin=self.spin, precision=self.precision)
         return out, served
 
-    def judge(self, r, J, served, nsample=100_000, seed=7):
+    def audit(self, r, J, served, nsample=100_000, seed=7):
         """exact-kernel check of a random sample of the served points:
         relative error sqrt(sum |dJ|^2 / sum |J|^2) and the worst cell"""
         import torch

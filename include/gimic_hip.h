@@ -288,7 +288,7 @@ int gimic_quadrature(double a, double b, int order, int npts, int kind, double *
  * tensors from the exact kernel (BF16 MFMA, FP32 master weights, Adam).
  * Loss: mean over points of |y - t|^2 / (|t|^2 + eps^2), t = J / out_scale.
  * Device pointers for points and tensors; the exact kernel stays the
- * fallback and the judge (see gimic_amd/surrogate.py).
+ * fallback and the audit of its error (see gimic_amd/surrogate.py).
  * ------------------------------------------------------------------------- */
 typedef struct gimic_mlp gimic_mlp;
 

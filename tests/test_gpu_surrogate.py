@@ -173,10 +173,10 @@ def test_surrogate_gains_over_exact_kernel():
     torch.cuda.synchronize()
     t_hybrid = time.perf_counter() - t0
     frac = float(served.float().mean())
-    judge = sg.judge(grid, J, served)
+    audit = sg.audit(grid, J, served)
     print("cells served", st["cells_served"], "grid served %.3f" % frac, "exact %.3f s hybrid %.3f s"
-          % (t_exact, t_hybrid), "judge", judge)
+          % (t_exact, t_hybrid), "audit", audit)
     assert frac > 0.05
-    assert judge["rel_err"] <= 5e-2
+    assert audit["rel_err"] <= 5e-2
     assert t_hybrid < t_exact
     ctx.close()
