@@ -1027,6 +1027,17 @@ int gimic_ctx_atoms(const gimic_ctx *c, double *coords, double *charges) {
     });
 }
 
+int gimic_ctx_atom_symbols(const gimic_ctx *c, char *sym) {
+    return guarded([&] {
+        if (!c || !sym) throw Error(GIMIC_EINVAL, "null argument");
+        for (int a = 0; a < c->basis.natoms(); a++) {
+            const std::string &s = a < (int)c->basis.symbols.size() ? c->basis.symbols[a] : "X ";
+            sym[2 * a] = s.size() > 0 ? s[0] : ' ';
+            sym[2 * a + 1] = s.size() > 1 ? s[1] : ' ';
+        }
+    });
+}
+
 int gimic_ctx_set_timing(gimic_ctx *c, int on) {
     return guarded([&] {
         if (!c) throw Error(GIMIC_EINVAL, "null context");

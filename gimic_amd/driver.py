@@ -7,12 +7,18 @@ and runs
                  current flux, per spin case (total; UHF adds alpha, beta,
                  spindens), then ACID when Essential.acid is on
                  -- src/fgimic/gimic.F90:222-261, integral.f90
-  calc=cdens     J B on every grid point (jfield.f90:62-184); with
-                 Essential.prop, the shielding of every nucleus and the
+  calc=cdens     J B on every grid point (jfield.f90:62-184) per spin case,
+                 written as the reference writes it (jvector_plots,
+                 jfield.f90:276-445): jvec[tag].vti on even grids,
+                 jmod[tag].vti (signed modulus, Essential.jmod) and acid.vti
+                 (Essential.acid) on 3-D grids, jmod[tag].txt on Gauss/Lobatto
+                 grids, jvec.vtu on file grids when grid.1.ele is present;
+                 with Essential.prop, the shielding of every nucleus and the
                  magnetizability on the external numgrid grid in coord.au,
                  gridfile.grd, grid_w.grd and nelpts.info (jfield.f90:584-929)
-printing the reference's stdout blocks.  All integrals come from one tensor
-evaluation per spin case (the reference evaluates J once per integral).
+printing the reference's stdout blocks; grid.xyz (plot_grid_xyz) in both
+modes.  All integrals come from one tensor evaluation per spin case (the
+reference evaluates J once per integral).
 
     python -m gimic_amd.driver gimic.inp [--precision fp64|mixed]
 """
@@ -22,7 +28,7 @@ import sys
 
 import numpy as np
 
-from . import flux, grid as gridmod, inp
+from . import flux, grid as gridmod, inp, writers
 from .lib import Context
 
 
@@ -88,13 +94,29 @@ def run_integral(kw, ctx, g, b, uhf, precision="fp64", out=sys.stdout):
     return res
 
 
-def run_cdens(kw, ctx, g, b, uhf, precision="fp64"):
+def run_cdens(kw, ctx, g, b, uhf, precision="fp64", outdir=None):
+    """J at every grid point per spin case (total; UHF adds alpha, beta and
+    spindens, gimic.F90:196-220); with outdir, each case's files are written
+    there as jvector_plots writes them (tags '', 'alpha', 'beta',
+    'spindens').  Returns {spin: (points, 9) tensors}."""
     pts = g.points()
+    ess = kw["Essential"]
+    elements = None
+    if outdir is not None and g.mode == "file":
+        ele = os.path.join(outdir, "grid.1.ele")
+        if os.path.exists(ele):
+            elements = writers.read_elements(ele)
+        else:
+            print("not writing a vtu file, because the file grid.1.ele was not found.")
     out = {}
     for spin in spin_cases(uhf):
-        J = ctx.jtensor(pts, spin=spin, precision=precision)
-        out[spin] = np.einsum("pbm,b->pm", J.reshape(-1, 3, 3), b)
-    return pts, out
+        out[spin] = ctx.jtensor(pts, spin=spin, precision=precision)
+        if outdir is not None:
+            writers.jvector_plots(g, out[spin], b, tag="" if spin == "total" else spin,
+                                  jmod=bool(ess.get("jmod", False)),
+                                  acid=bool(ess.get("acid", False)), outdir=outdir,
+                                  elements=elements)
+    return out
 
 
 def _fortran_a(text, w):
@@ -192,12 +214,12 @@ def main(argv=None):
     a = ap.parse_args(argv)
     kw = inp.parse(open(a.input).read())
     kw, ctx, g, b, uhf = setup(kw, os.path.dirname(os.path.abspath(a.input)))
+    coords, _ = ctx.atoms()
+    writers.write_grid_xyz("grid.xyz", g, ctx.symbols(), coords)
     if kw.get("calc") == "integral":
         run_integral(kw, ctx, g, b, uhf, a.precision)
     else:
-        pts, vec = run_cdens(kw, ctx, g, b, uhf, a.precision)
-        for spin, v in vec.items():
-            np.savetxt(f"jvec_{spin}.txt", np.hstack([pts, v]), fmt="%.10e")
+        run_cdens(kw, ctx, g, b, uhf, a.precision, outdir=".")
         if kw["Essential"].get("prop", False):
             run_property(ctx, os.path.dirname(os.path.abspath(a.input)), precision=a.precision)
     ctx.close()

@@ -19,7 +19,7 @@ EXPORTS = [
     "gimic_set_screening", "gimic_calc_jtensor", "gimic_calc_jvector", "gimic_calc_modj",
     "gimic_get_gauss_points", "mkgausspoints", "gimic_last_error", "gimic_ctx_create_files",
     "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
-    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_tuning_log", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_tile_stats", "gimic_ctx_tile_stats", "gimic_ctx_set_fp64_radius", "gimic_ctx_fp64_refined", "gimic_ctx_set_chunk_bytes", "gimic_ctx_chunk_bytes", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
+    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_atom_symbols", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_tuning_log", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_tile_stats", "gimic_ctx_tile_stats", "gimic_ctx_set_fp64_radius", "gimic_ctx_fp64_refined", "gimic_ctx_set_chunk_bytes", "gimic_ctx_chunk_bytes", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
     "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_flux_plane_grids", "gimic_ctx_properties", "gimic_sph_transform", "gimic_quadrature",
     "gimic_selftest_mfma_layout", "gimic_selftest_mfma32_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_set_stream", "gimic_mlp_info",
     "gimic_mlp_params", "gimic_mlp_loss_grad", "gimic_mlp_train_step", "gimic_mlp_predict",
@@ -72,6 +72,7 @@ def load():
         "gimic_ctx_info": [P, P, P, P, P, P],
         "gimic_ctx_destroy": [P],
         "gimic_ctx_atoms": [P, P, P],
+        "gimic_ctx_atom_symbols": [P, P],
         "gimic_ctx_set_timing": [P, I],
         "gimic_ctx_get_timing": [P, P, P],
         "gimic_ctx_tuning": [P, P, P, P],
@@ -347,6 +348,14 @@ class Context:
         q = np.zeros(n)
         check(self._lib.gimic_ctx_atoms(self._h, ptr(c), ptr(q)))
         return c, q
+
+    def symbols(self):
+        """two-character atom symbols as the MOL file gives them"""
+        n = self.info()["natoms"]
+        buf = ctypes.create_string_buffer(2 * n)
+        check(self._lib.gimic_ctx_atom_symbols(self._h, buf))
+        raw = buf.raw.decode(errors="replace")
+        return [raw[2 * a:2 * a + 2] for a in range(n)]
 
     def set_density(self, dens):
         """dens: (nset, 4, n, n) with [.., mu, nu] = D(mu, nu)."""

@@ -116,6 +116,11 @@ int gimic_ctx_info(const gimic_ctx *ctx, int *nbf, int *nbf_cart, int *natoms, i
  * (intgrl.f90:91-115); either pointer may be NULL */
 int gimic_ctx_atoms(const gimic_ctx *ctx, double *coords, double *charges);
 
+/* atom symbols as the MOL file's atom lines give them (the first two
+ * characters, intgrl.f90:111-112; "X " for a basis built from arrays):
+ * 2 characters per atom, not NUL-terminated, into sym[2*natoms] */
+int gimic_ctx_atom_symbols(const gimic_ctx *ctx, char *sym);
+
 /* Per-kernel timing (no counterpart in the reference): when on, every
  * launch of the basis-evaluation, contraction and finalize kernels is
  * bracketed by HIP events on its stream; gimic_ctx_get_timing waits for

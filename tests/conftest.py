@@ -1,4 +1,6 @@
+import lzma
 import os
+import re
 import sys
 
 import numpy as np
@@ -53,3 +55,35 @@ def c4h4_files(tmp_path_factory):
 @pytest.fixture(scope="session")
 def openshell_files(tmp_path_factory):
     return write_fixture(str(tmp_path_factory.mktemp("openshell")), "openshell.npz")
+
+
+def writer_fixture(name):
+    """text of one of the reference's output files held in writers.npz
+    (name: its path under the reference's test/ directory)"""
+    d = np.load(os.path.join(GOLDEN, "writers.npz"))
+    return lzma.decompress(d[name].tobytes()).decode()
+
+
+_NUM = re.compile(r"[-+]?(?:\d+\.\d*|\.\d+|\d+)(?:[eEdD][-+]?\d+)?")
+
+
+def runtest_mismatches(out_text, ref_text, rel_tolerance):
+    """Whole-file comparison with runtest's numeric-filter semantics: every
+    number in both files in order, integers exactly, reals to a relative
+    tolerance (a zero reference value must be matched exactly).  Returns the
+    (written, reference) token pairs that fail; raises when the files hold
+    different counts of numbers."""
+    a, r = _NUM.findall(out_text), _NUM.findall(ref_text)
+    if len(a) != len(r):
+        raise AssertionError(f"{len(a)} numbers written, {len(r)} in the reference file")
+    bad = []
+    for x, y in zip(a, r):
+        if x == y:
+            continue
+        if not re.search(r"[.eEdD]", y):
+            bad.append((x, y))
+            continue
+        fx, fy = float(x.replace("D", "E")), float(y.replace("D", "E"))
+        if fy == 0.0 or abs(fx - fy) > rel_tolerance * abs(fy):
+            bad.append((x, y))
+    return bad

@@ -4,16 +4,22 @@ the reference tree; the GPU box only reads the files written here).
   c4h4.npz        test/c4h4/{MOL,XDENS} (basis text + density values as
                   stored), read-grid points and reference jvec.vtu vectors,
                   integration/reference/stdout flux numbers
-  openshell.npz   test/open-shell/{MOL,XDENS}, integration and 3d reference
-                  outputs (stdout numbers, jvec*.vti vectors)
+  openshell.npz   test/open-shell/{MOL,XDENS}, integration and 3d inputs and
+                  the integration reference stdout
+  writers.npz     the reference's own output files that pin the field
+                  writers, as the test data holds them (xz-compressed
+                  bytes): open-shell/3d/reference/{jvec,jmod}[tag].vti,
+                  c4h4/read-grid/reference/jvec.vtu, and the TetGen cells
+                  c4h4/read-grid/grid.1.ele that run reads
   ring12_ref.npz  flang-built reference (oracle/_ref) tensors, all nine
                   components, for the synthetic 12-atom ring at 256 seeded
                   points, the cartesian-equivalent density it was fed, and
                   composite Gauss/Lobatto rules from the reference's
                   setup_gauss_data
 
-Usage: python tests/golden/make_golden.py [/root/reference]
+Usage: python tests/golden/make_golden.py [/root/reference] [--writers-only]
 """
+import lzma
 import os
 import re
 import sys
@@ -65,6 +71,20 @@ def flux_block(stdout, marker):
     return numbers("\n".join(lines[i:i + 10]))
 
 
+def writer_files(ref):
+    """writers.npz: the reference's output files named in the module
+    docstring, each as xz-compressed bytes under its path in test/"""
+    t = os.path.join(ref, "test")
+    names = ["open-shell/3d/reference/%s%s.vti" % (k, tag) for k in ("jvec", "jmod")
+             for tag in ("", "alpha", "beta", "spindens")]
+    names += ["c4h4/read-grid/reference/jvec.vtu", "c4h4/read-grid/grid.1.ele"]
+    out = {}
+    for n in names:
+        raw = open(os.path.join(t, n), "rb").read()
+        out[n] = np.frombuffer(lzma.compress(raw, preset=9 | lzma.PRESET_EXTREME), dtype=np.uint8)
+    np.savez(os.path.join(HERE, "writers.npz"), **out)
+
+
 def main(ref):
     t = os.path.join(ref, "test")
     # ---- c4h4 ----
@@ -90,11 +110,8 @@ def main(ref):
                                                       "stdout")).read()),
         threed_inp=np.array(open(os.path.join(d, "3d", "gimic.inp")).read()),
     )
-    for name in ["jvec", "jvecalpha", "jvecbeta", "jvecspindens"]:
-        ext, org, spc, vals = vti_data(os.path.join(d, "3d", "reference", name + ".vti"))
-        out[name] = vals.astype(np.float32)  # printed with 6 digits (e14.6)
-        out["vti_extent"], out["vti_origin"], out["vti_spacing"] = ext, org, spc
     np.savez_compressed(os.path.join(HERE, "openshell.npz"), **out)
+    writer_files(ref)
     # ---- synthetic ring through the flang-built reference ----
     from gimic_amd import synth
     from oracle import molio, refrun, sph
@@ -133,4 +150,9 @@ def main(ref):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    ref = args[0] if args else "/root/reference"
+    if "--writers-only" in sys.argv:
+        writer_files(ref)
+    else:
+        main(ref)

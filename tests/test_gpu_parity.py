@@ -103,25 +103,6 @@ def test_openshell_spin_cases_match_oracle(openshell, spin):
     assert rel_err(J, Jo).max() <= FP64_TOL
 
 
-def test_openshell_3d_fixture(openshell):
-    """test/open-shell/3d: 33^3 even grid on [-8,8]^3, B = +z, jvec*.vti (e14.6)"""
-    ctx, _, _ = openshell
-    d = golden("openshell.npz")
-    n = 33
-    ax = -8.0 + 0.5 * np.arange(n)
-    z, y, x = np.meshgrid(ax, ax, ax, indexing="ij")
-    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
-    for spin, key in [("total", "jvec"), ("alpha", "jvecalpha"), ("beta", "jvecbeta"),
-                      ("spindens", "jvecspindens")]:
-        print(f"case open-shell 3d {spin}", flush=True)
-        J = ctx.jtensor(pts, spin=spin)
-        jv = J[:, 6:9].ravel()
-        ref = d[key].astype(np.float64)
-        # e14.6 keeps six significant digits: rounding error up to 5e-6 relative
-        ok = np.abs(jv - ref) <= 5.1e-6 * np.abs(ref) + 1e-16
-        assert ok.mean() == 1.0, (spin, np.abs(jv - ref).max())
-
-
 @pytest.mark.parametrize("npts", [0, 1, 127, 129, 1000])
 def test_ragged_batch_sizes(ring12, npts):
     s, ctx = ring12

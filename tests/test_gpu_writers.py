@@ -1,0 +1,96 @@
+"""The fgimic-compatible driver on the GPU writes the reference's field files.
+
+`python -m gimic_amd.driver gimic.inp` (driver.main, run in a scratch
+directory) with the reference's own test inputs:
+  * test/open-shell/3d (33^3 even grid, UHF, jmod=on): jvec*.vti and
+    jmod*.vti of all four spin cases against the reference's files with
+    runtest's comparison at rel 1e-7, as the reference's test script sets it;
+    with acid=on also acid.vti, against the same file written from the
+    oracle's tensors;
+  * test/c4h4/read-grid (file grid, B = (0, 0, -1)): jvec.vtu at rel 1e-8;
+  * grid.xyz in both runs.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden, runtest_mismatches, write_fixture, writer_fixture
+from gimic_amd import driver, writers
+from oracle import molio, oracle
+
+pytestmark = pytest.mark.gpu
+
+READGRID_INP = """calc=cdens
+basis="MOL"
+xdens="XDENS"
+openshell=false
+magnet=[0.0, 0.0, -1.0]
+Grid(file) {
+    file=gridfile.grd
+}
+Advanced {
+    spherical=off
+    diamag=on
+    paramag=on
+    GIAO=on
+    screening=on
+    screening_thrs=1.d-8
+}
+"""
+
+
+def _run_driver(tmp_path, monkeypatch, inp_text):
+    (tmp_path / "gimic.inp").write_text(inp_text)
+    monkeypatch.chdir(tmp_path)
+    driver.main([str(tmp_path / "gimic.inp")])
+
+
+def test_driver_writes_openshell_3d_files(tmp_path, monkeypatch):
+    write_fixture(str(tmp_path), "openshell.npz")
+    inp_text = str(golden("openshell.npz")["threed_inp"])
+    print("case driver open-shell/3d cdens", flush=True)
+    _run_driver(tmp_path, monkeypatch, inp_text)
+    for tag in ("", "alpha", "beta", "spindens"):
+        for name in (f"jvec{tag}.vti", f"jmod{tag}.vti"):
+            out = (tmp_path / name).read_text()
+            ref = writer_fixture(f"open-shell/3d/reference/{name}")
+            assert runtest_mismatches(out, ref, 1e-7) == [], name
+    assert not (tmp_path / "acid.vti").exists()
+    lines = (tmp_path / "grid.xyz").read_text().splitlines()
+    assert int(lines[0]) == len(lines) - 2
+
+
+def test_driver_acid_vti_matches_oracle_written_file(tmp_path, monkeypatch):
+    mol, xd = write_fixture(str(tmp_path), "openshell.npz")
+    inp_text = str(golden("openshell.npz")["threed_inp"]).replace("acid=off", "acid=on")
+    assert "acid=on" in inp_text
+    print("case driver open-shell/3d acid.vti", flush=True)
+    _run_driver(tmp_path, monkeypatch, inp_text)
+    # every spin case rewrites acid.vti (jfield.f90:388-392): the last,
+    # spindens, is what stays
+    from gimic_amd import grid as gridmod, inp
+    b = molio.read_mol(mol)
+    dens = molio.read_xdens(xd, b, uhf=True)
+    kw = inp.with_defaults(inp.parse(inp_text))
+    g = gridmod.new_grid(dict(kw["Grid"]), b.coords)
+    J = oracle.jtensor(b.arrays(), dens, g.points(), screening=1e-8, spin="spindens")
+    ref_dir = tmp_path / "oracle_files"
+    ref_dir.mkdir()
+    writers.write_vtk_imagedata(str(ref_dir / "acid.vti"), g, writers.acid_values(J))
+    out = (tmp_path / "acid.vti").read_text()
+    ref = (ref_dir / "acid.vti").read_text()
+    assert runtest_mismatches(out, ref, 1e-7) == []
+
+
+def test_driver_writes_readgrid_vtu(tmp_path, monkeypatch):
+    write_fixture(str(tmp_path), "c4h4.npz")
+    d = golden("c4h4.npz")
+    np.savetxt(tmp_path / "gridfile.grd", d["readgrid_points"], fmt="%.17g")
+    (tmp_path / "grid.1.ele").write_text(writer_fixture("c4h4/read-grid/grid.1.ele"))
+    print("case driver c4h4/read-grid cdens", flush=True)
+    _run_driver(tmp_path, monkeypatch, READGRID_INP)
+    out = (tmp_path / "jvec.vtu").read_text()
+    ref = writer_fixture("c4h4/read-grid/reference/jvec.vtu")
+    assert runtest_mismatches(out, ref, 1e-8) == []
+    assert (tmp_path / "grid.xyz").exists()
