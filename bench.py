@@ -110,7 +110,7 @@ def kernel_roofline(ctx, d_pts, out, precision, nbf):
     achieved = flop / launches / avg_ms / 1e9  # TFLOP/s per launch average
     return dict(bound="mfma", achieved=round(achieved, 2), peak=PEAK_TFLOPS[precision],
                 unit="TFLOP/s", frac=round(achieved / PEAK_TFLOPS[precision], 4), traffic=None,
-                kernel="contract_bf16x6_kernel" if precision != "fp64" else "contract_f64_kernel",
+                kernel="contract_mx2_kernel" if precision != "fp64" else "contract_f64_kernel",
                 avg_launch_ms=round(avg_ms, 3), launches=launches,
                 flop_per_launch=flop / launches, dense_flop_per_launch=dense / launches,
                 listed_block_pairs=round(block_pairs, 4),
@@ -137,7 +137,11 @@ def cpu_baseline(system, budget_s=15.0):
     from oracle import molio, refrun, sph
     if not refrun.available():
         return None
-    threads = min(16, os.cpu_count() or 1)
+    # every core this process may use: the OpenMP thread count the host
+    # sets for it (a GPU host gives one GPU's job its share of the cores),
+    # else the cores in its affinity mask
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)) or 1)
     b = molio.Basis()
     b.symbols, b.charges, b.coords = system.symbols, system.charges, system.coords
     b.shell_atom, b.shell_l, b.shell_nprim, b.shell_poff = (system.shell_atom, system.shell_l,
@@ -168,28 +172,11 @@ def cpu_baseline(system, budget_s=15.0):
             _, t = refrun.ref_jtensor(mol, sample, dens=flat, screening=1e-8, threads=threads,
                                       return_time=True)
     return {"value": round(n / t, 2), "unit": "points/s", "cores": threads, "kind": "reference",
+            "nproc": nproc, "affinity_cores": len(os.sched_getaffinity(0)),
             "sample": f"{n} random points of the 128^3 grid, flang-built reference "
-                      f"(oracle/_ref/ref_driver, OpenMP over points), cartesian-equivalent "
-                      f"density D_cart = T^T D T, loop time {t:.2f} s"}
-
-
-def scan_planes(system, nplanes=256, seed=7):
-    """planes perpendicular to the bond of atoms 1-2, offsets -6..+6 bohr from
-    the bond midpoint, each with its own order-7 Lobatto grid of n1 x n2
-    points, n1 in [40, 120], n2 in [60, 200] (seeded, rounded up to the order)"""
-    from gimic_amd import flux, grid as gridmod
-    c = np.asarray(system.coords)
-    half = float(np.linalg.norm(c[1] - c[0]) / 2)
-    rng = np.random.default_rng(seed)
-    planes = []
-    for off in np.linspace(-6.0, 6.0, nplanes):
-        n1, n2 = int(rng.integers(40, 121)), int(rng.integers(60, 201))
-        kw = {"_arg": "bond", "type": "lobatto", "gauss_order": 7, "bond": [1, 2],
-              "fixcoord": [0.0, 0.0, 10.0], "distance": half + float(off),
-              "height": [-8.0, 8.0], "width": [-10.0, 10.0], "grid_points": [n1, n2, 0]}
-        g = gridmod.new_grid(kw, c)
-        planes.append((g, gridmod.get_magnet(g, "X")))
-    return planes
+                      f"(oracle/_ref/ref_driver, -O3 -fopenmp, OpenMP over points on {threads} "
+                      f"threads), cartesian-equivalent density D_cart = T^T D T, loop time "
+                      f"{t:.2f} s"}
 
 
 def bench_scan(args):
@@ -201,7 +188,8 @@ def bench_scan(args):
     from gimic_amd import flux, synth
     system = synth.sheet30()
     ctx = system.context(screening=1e-8)
-    scan = scan_planes(system)
+    ctx.set_chunk_bytes(-1)  # the large chunk: a third of device memory
+    scan = synth.scan_planes(system)
     grids, fields = [g for g, _ in scan], [b for _, b in scan]
     planes = [flux.plane_of(g, b) for g, b in scan]
     npts = sum(len(p["weights"]) for p in planes)
@@ -252,6 +240,7 @@ def bench_openshell(args):
     from gimic_amd import synth
     system = synth.sheet200()
     ctx = system.context(screening=1e-8)
+    ctx.set_chunk_bytes(-1)  # the large chunk: a third of device memory
     pts = np.stack(np.meshgrid(0.2 * np.arange(192) - 19.1, 0.2 * np.arange(192) - 19.1,
                                0.2 * np.arange(96) - 9.5, indexing="ij"), -1)
     pts = pts.transpose(2, 1, 0, 3).reshape(-1, 3)  # x fastest
@@ -285,7 +274,7 @@ def bench_openshell(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms["mixed"], 3),
         "higher_is_better": True, "dtype": "fp32 (bf16x6 MFMA), fp64 assembly",
         "data": "synthetic",
-        "config": {"workload": "sheet200 C166H34, nbf 2494, alpha/beta nocc 610/600, "
+        "config": {"workload": "sheet200 C180H20, nbf 2620, alpha/beta nocc 610/600, "
                                "192x192x96 grid at 0.2 bohr", "points_per_step": npts,
                    "nbf": system.nbf},
         "fp64_gpu": {"value": round(npts / (ms["fp64"] / 1e3), 1), "ms_per_step": round(ms["fp64"], 3)},
@@ -318,6 +307,7 @@ def bench_property(args):
     from gimic_amd import synth
     system = synth.cage60()
     ctx = system.context(screening=1e-8)
+    ctx.set_chunk_bytes(-1)  # the large chunk: a third of device memory
     coords = np.asarray(system.coords)
     pts, w, sizes = atom_product_grids(coords)
     npts = len(pts)
@@ -372,6 +362,7 @@ def bench_surrogate(args):
     from gimic_amd.surrogate import Surrogate
     system = synth.cage60()
     ctx = system.context(screening=1e-8)
+    ctx.set_chunk_bytes(-1)  # the large chunk: a third of device memory
     half = 7.68
     # the network covers points farther than `exclusion` bohr from every
     # nucleus (the field there is smooth enough to learn); nearer points and
@@ -462,6 +453,9 @@ def main():
         pts = pts[:args.points]
     npts = len(pts)
     ctx = system.context(screening=1e-8)
+    # the large chunk (a third of device memory): the whole grid in one
+    # mixed launch; the library's default is a modest 8 GiB
+    ctx.set_chunk_bytes(-1)
     d_pts = torch.tensor(pts, dtype=torch.float64, device=dev)
     out = torch.empty((npts, 9), dtype=torch.float64, device=dev)
 
@@ -472,10 +466,8 @@ def main():
     roof, ktimes = kernel_roofline(ctx, d_pts, out, "mixed", nbf)
     variant, split, tune_ms = ctx.tuning()
     tuning_log = [[v, sp, round(t, 3)] for v, sp, t in ctx.tuning_log()]
-    roof["kernel"] = {1: "contract_bf16x6_kernel", 2: "contract_mx_kernel",
-                      3: "contract_mx2_kernel<2,2,4>", 4: "contract_mx2_kernel<3,2,4>",
-                      5: "contract_mx2_kernel<2,4,4>", 6: "contract_mx3_kernel<2>",
-                      7: "contract_mx3p_kernel", 8: "contract_mx2_kernel<2,2,8>",
+    roof["kernel"] = {3: "contract_mx2_kernel<2,2,4>", 4: "contract_mx2_kernel<3,2,4>",
+                      5: "contract_mx2_kernel<2,4,4>", 8: "contract_mx2_kernel<2,2,8>",
                       9: "contract_mx2_kernel<3,2,8>"}.get(variant, "?")
     roof["split"] = split
     # the mixed kernels issue six BF16 MFMA products per algorithmic product

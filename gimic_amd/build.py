@@ -11,7 +11,8 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libgimic_hip.so")
 ARCH = os.environ.get("GIMIC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["basis_eval.hip", "contract_f64.hip", "contract_bf16.hip", "contract_mx.hip", "contract_mx2.hip", "contract_mx3.hip", "finalize.hip", "selftest.hip", "surrogate.hip", "tiles.hip", "planes.hip", "refine.hip",
+SOURCES = ["basis_eval.hip", "contract_f64.hip", "contract_mx2.hip", "finalize.hip", "selftest.hip",
+           "surrogate.hip", "tiles.hip", "planes.hip", "refine.hip",
            "api.cpp", "surrogate_api.cpp", "host_basis.cpp", "quadrature.cpp"]
 
 # Packed FP32 VALU ops stay disabled in device code.

@@ -60,18 +60,27 @@ int guarded(F &&f) {
     }
 }
 
-// Owned device allocation.
+// Device buffer: an owned allocation, or (view) a range of memory owned
+// elsewhere, such as the device scratch of the allocation-free entry point.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool own = true;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && own) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        own = true;
+    }
+    void view(void *q, size_t n) {
+        release();
+        p = q;
+        bytes = n;
+        own = false;
     }
     void alloc(size_t n, hipStream_t s, bool zero = true) {
         if (n <= bytes && p) {
@@ -90,11 +99,50 @@ struct DevBuf {
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
 // Device memory for the basis values and gradients of one chunk of points
-// (one pass of basis evaluation, contraction and assembly): a third of the
-// device's memory, at most kChunkBytesMax.  The total, not the free memory,
-// sets it, so the chunks (and with them the results' last bits, which the
+// (one pass of basis evaluation, contraction and assembly).  The default is
+// a fixed 8 GiB, modest beside other users of the device; the large setting
+// (gimic_ctx_set_chunk_bytes(ctx, -1)) is a third of the device's memory,
+// at most kChunkBytesMax.  The total, not the free memory, sets the large
+// one, so the chunks (and with them the results' last bits, which the
 // tiles' block lists touch) are the same on every run on one device model.
+constexpr size_t kChunkBytesDefault = (size_t)8 << 30;
 constexpr size_t kChunkBytesMax = (size_t)96 << 30;
+
+// Bump allocator over a device range (the scratch of the allocation-free
+// entry point): carve() hands out 256-byte aligned views and clears them
+// with a kernel when asked (no hipMemsetAsync, so the calls may sit in a
+// captured graph); nothing is freed until reset().
+struct Arena {
+    unsigned char *base = nullptr;
+    size_t size = 0, used = 0;
+    void reset(size_t to = 0) { used = to; }
+    void carve(DevBuf &b, size_t bytes, bool zero, hipStream_t st);
+};
+
+// The work buffers of one evaluation: the context's own (grown as needed and
+// kept), or views carved from an Arena.
+struct Workspace {
+    DevBuf phi64, grad64, phis, grad32, part;
+    // spatial tiles of a chunk: sorted points, their caller indices, sort
+    // scratch, and the screening block list of every contraction row tile
+    DevBuf spts, perm, bpart, sortscr, bcount, blist, toff, k8count, k8list, bzero;
+    // FP64 pass near nuclei: selection index list, their points, selection
+    // scratch and count
+    DevBuf near_idx, near_pts, near_scr, near_cnt;
+};
+
+void Arena::carve(DevBuf &b, size_t bytes, bool zero, hipStream_t st) {
+    if (!(b.p && !b.own && b.bytes >= bytes)) {
+        const size_t off = round_up(used, 256), n = round_up(std::max<size_t>(bytes, 1), 256);
+        if (off > size || n > size - off)
+            throw Error(GIMIC_ENOMEM, "device scratch exhausted (" + std::to_string(size >> 20) +
+                                          " MiB)");
+        b.view(base + off, bytes);
+        used = off + n;
+    }
+    if (zero && bytes > 0)
+        hip_check(launch_fill_zero((double *)b.p, round_up(bytes, 8) / 8, st), "fill_zero");
+}
 
 // Expected cost of a function order: the mean over sample points of
 // (number of live 32-function blocks)^2, the live (K-block, N-block) pairs
@@ -213,7 +261,7 @@ struct gimic_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     size_t chunk_bytes = 0;  // basis buffers of one chunk of points
-    size_t chunk_bytes_default = 0;
+    size_t chunk_bytes_large = 0;  // the large setting: a third of the device memory
     Basis basis;
     double screen_thrs = 1e-6;
     int giao = 1, diamag = 1, paramag = 1;
@@ -223,45 +271,38 @@ struct gimic_ctx {
     DevBuf shells, exps, ncc, coords, rf64, rf32;
     // right operands per (spin list, precision)
     struct Mats {
-        DevBuf b64, b32;
+        DevBuf b64;
         DevBuf bs3;  // BF16 parts in LDS-image blocks (pre-split mixed kernel)
         int nmatset = 0;
     };
     std::map<std::pair<std::vector<int>, int>, std::unique_ptr<Mats>> mats;
-    // workspace
-    DevBuf phi64, grad64, phi32, grad32, phis, part, pts_tmp, jt_tmp, contrib, planes,
-        pparams, offs, outs;
+    // evaluation workspace, and host-array staging of the batch calls
+    Workspace ws;
+    DevBuf pts_tmp, jt_tmp, contrib, planes, pparams, offs, outs;
     // internal function order (see setup_device): fun_orig[i] is the caller's
     // index of internal function i; per 32-function block the shells with a
     // function in it (blk_beg ranges into blk_sh)
     std::vector<int> fun_orig;
     DevBuf blk_beg, blk_sh;
-    // spatial tiles of a chunk: sorted points, their caller indices, sort
-    // scratch, and the screening block list of every contraction row tile
-    DevBuf spts, perm, bpart, sortscr, bcount, blist, toff, k8count, k8list, bzero;
     // property quadrature (gimic_ctx_properties): points + weights, tensors,
     // partition offsets, nuclei, sums
     DevBuf prop_in, prop_jt, prop_offs, prop_nuc, prop_out;
     // plane-grid descriptors (gimic_flux_plane_grids)
     DevBuf gspec;
     // mixed precision: points within fp64_radius bohr of a nucleus are
-    // evaluated again in FP64 (0: off); selection index list, their points,
-    // selection scratch and count; points refined by the last mixed call
+    // evaluated again in FP64 (0: off); points refined by the last mixed call
     double fp64_radius = 0.4;
-    DevBuf near_idx, near_pts, near_scr, near_cnt;
     long long refined_last = 0;
     // optional per-kernel timing with HIP events on the launch stream:
     // [0] basis evaluation, [1] contraction, [2] finalize
     bool timing = false;
     // mixed-precision contraction kernel chosen by timing (-1: not yet tuned;
-    // 1 register-tiled, 2 LDS-tiled, 3 and 4 pre-split operands with
-    // direct-to-LDS staging, double or triple buffered, 5 pre-split with 256
-    // rows per workgroup, 6 and 7 pre-split on 32x32 MFMA blocks, double
-    // buffered or pipelined, 8 and 9 pre-split with 8 wavefronts and 256
-    // rows, double or triple buffered) and the timings (basis evaluation +
-    // contraction) that chose it.  Variants 0 (FP64), 3-5, 8 and 9 walk the
-    // screening block lists; 1, 2, 6 and 7 compute the dense product and are
-    // not among the tuner's candidates (set_mixed_kernel selects them)
+    // pre-split operands with direct-to-LDS staging: 3 and 4 with 4
+    // wavefronts and 128 rows, double or triple buffered, 5 with 4
+    // wavefronts and 256 rows, 8 and 9 with 8 wavefronts and 256 rows,
+    // double or triple buffered) and the timings (basis evaluation +
+    // contraction) that chose it.  Every variant, and the FP64 kernel
+    // (variant 0 here), walks the screening block lists
     int mixed_variant = -1;
     int mixed_split = 0;  // N-split count of the pre-split kernel (0: by workgroup count)
     // block-list statistics (gimic_ctx_set_tile_stats): tiles, sum n, sum n^2, rows per tile
@@ -301,9 +342,9 @@ void setup_device(gimic_ctx *c) {
     hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
     size_t mem_free = 0, mem_total = 0;
     hip_check(hipMemGetInfo(&mem_free, &mem_total), "hipMemGetInfo");
-    c->chunk_bytes_default =
+    c->chunk_bytes_large =
         std::max<size_t>((size_t)1 << 30, std::min(kChunkBytesMax, mem_total / 3));
-    c->chunk_bytes = c->chunk_bytes_default;
+    c->chunk_bytes = kChunkBytesDefault;
     Basis &b = c->basis;
     finalize_basis(b, c->screen_thrs);
     c->lda = (int)round_up((size_t)b.nbf, kKPad);
@@ -486,11 +527,7 @@ gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec)
                   "hipMemcpyAsync");
     } else {
         std::vector<float> b32(B.begin(), B.end());
-        m->b32.alloc(b32.size() * sizeof(float), st, false);
-        hip_check(hipMemcpyAsync(m->b32.p, b32.data(), b32.size() * sizeof(float),
-                                 hipMemcpyHostToDevice, st),
-                  "hipMemcpyAsync");
-        // the same FP32 values as three BF16 parts (x = h + m + l exactly),
+        // the FP32 values as three BF16 parts (x = h + m + l exactly),
         // one 8-function group per (set, N-tile, group kg), [part][column][8 k],
         // column = matrix * 32 + function within the N-tile; the pre-split
         // kernels gather four groups per K-step into their LDS buffers
@@ -569,28 +606,93 @@ enum class EvalMode {
     Eager,    // allocate as needed, tune, launch
     Reserve,  // allocate every buffer the launches need and tune; launch nothing else
     Capture,  // kernel launches only (no allocation, memset, timing or sync)
+    Scratch,  // work buffers carved from an Arena: no allocation, tuning, timing or
+              // statistics (the allocation-free entry point)
 };
 
+// Where an evaluation's work buffers come from: the context's workspace
+// (ws null), another Workspace, and in EvalMode::Scratch views carved from
+// `arena`, at most max_rows rows per chunk.
+struct EvalWork {
+    Workspace *ws = nullptr;
+    Arena *arena = nullptr;
+    size_t max_rows = 0;
+};
+
+// The mixed kernel and N-split count of an untuned context in Scratch mode:
+// 8 wavefronts, triple buffered, four N-ranges (the tuner's usual choice on
+// large grids).  Scratch mode splits a row tile over at most
+// kScratchMaxSplit N-ranges, which bounds the partial sums it plans for.
+constexpr int kDefaultVariant = 9, kDefaultSplit = 4;
+constexpr int kScratchMaxSplit = 32;
+
+// Row tile of the mixed kernels: 128 rows (4 wavefronts x 32), 256 (4 x 64
+// or 8 x 32)
+int mixed_tile_rows(int variant) { return variant == 5 || variant >= 8 ? 256 : 128; }
+
+// Arena bytes one Scratch-mode chunk of `rows` rows carves (every carve
+// rounded to 256 bytes), with `split` N-ranges per row tile
+size_t scratch_eval_bytes(const gimic_ctx *c, int prec, int nmatset, size_t rows, int split) {
+    const size_t ld = (size_t)c->lda, nblk = ld / 32, t = rows / 128;
+    auto r = [](size_t b) { return round_up(std::max<size_t>(b, 1), 256); };
+    size_t n = prec == GIMIC_PREC_FP64 ? r(rows * ld * 8) + r(3 * rows * ld * 8)
+                                       : r(rows * ld * 6) + r(3 * rows * ld * 4);
+    n += r(rows * 24) + r(rows * 4) + r(256 * 6 * 8) + r((size_t)sort_scratch_ints((int)rows) * 4);
+    n += r(t * 4) + r(t * nblk * 4) + r((t + 1) * 4) + r(t * 4) + r(t * nblk * 16) + r(12 * 1024);
+    n += r((size_t)nmatset * split * kNQ * rows * 8);
+    return n;
+}
+
+// Largest chunk (a multiple of the precision's row granularity) whose
+// Scratch-mode buffers fit in `bytes`; 0 if not even one fits
+size_t scratch_rows(const gimic_ctx *c, int prec, int nmatset, size_t bytes, int split) {
+    const size_t gran = prec == GIMIC_PREC_FP64 ? (size_t)kWgRows : 256;
+    size_t lo = 0, hi = gran;
+    while (scratch_eval_bytes(c, prec, nmatset, hi, split) <= bytes) {
+        lo = hi;
+        hi *= 2;
+    }
+    while (hi - lo > gran) {
+        const size_t mid = round_up((lo + hi) / 2, gran);
+        if (mid >= hi) break;
+        if (scratch_eval_bytes(c, prec, nmatset, mid, split) <= bytes) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 void refine_near_nuclei(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
-                        const std::vector<int> &spins, hipStream_t st, const FluxDev *flux);
+                        const std::vector<int> &spins, hipStream_t st, const FluxDev *flux,
+                        EvalWork work = {});
 
 // Evaluate J for npts device points into d_jt (nspin blocks of 9*npts).
 // With out_idx (a subset run), point i's tensor and flux contribution go to
 // index out_idx[i] of output arrays sized for out_npts points.  A mixed
 // evaluation in eager mode ends with the FP64 pass over the points next to
-// a nucleus (refine_near_nuclei); a captured one leaves it to the caller.
+// a nucleus (refine_near_nuclei); a captured or Scratch-mode one leaves it
+// to the caller.
 void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
               const std::vector<int> &spins, int prec, hipStream_t st, const FluxDev *flux,
               EvalMode mode = EvalMode::Eager, const long long *out_idx = nullptr,
-              int64_t out_npts = 0) {
+              int64_t out_npts = 0, EvalWork work = {}) {
     if (npts <= 0) return;
     if (prec != GIMIC_PREC_FP64 && prec != GIMIC_PREC_MIXED)
         throw Error(GIMIC_EINVAL, "unknown precision");
     for (int s : spins)
         if (s < 0 || s > 3) throw Error(GIMIC_EINVAL, "invalid spin case");
     const bool capture = mode == EvalMode::Capture;
-    if (capture && !c->mats.count(std::make_pair(spins, prec)))
-        throw Error(GIMIC_ESTATE, "capture before reserve: right operands missing");
+    const bool scratch = mode == EvalMode::Scratch;
+    if ((capture || scratch) && !c->mats.count(std::make_pair(spins, prec)))
+        throw Error(GIMIC_ESTATE, capture ? "capture before reserve: right operands missing"
+                                          : "right operands of this spin case and precision "
+                                            "are not resident (one allocating call builds them)");
+    if (scratch && (!work.arena || work.max_rows == 0))
+        throw Error(GIMIC_EINVAL, "scratch evaluation without a planned chunk");
+    Workspace &W = work.ws ? *work.ws : c->ws;
+    // an eager mixed evaluation builds the FP64 operands of its FP64 pass
+    // near nuclei up front, so that a later allocation-free call finds both
+    if (prec == GIMIC_PREC_MIXED && mode == EvalMode::Eager && !out_idx && c->fp64_radius > 0.0)
+        (void)get_mats(c, spins, GIMIC_PREC_FP64);
     auto &M = get_mats(c, spins, prec);
     // per-kernel event timing only for eager evaluation
     struct TimingOff {
@@ -600,52 +702,89 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
     } timing_guard{c, c->timing};
     if (mode != EvalMode::Eager) c->timing = false;
     auto grow = [&](DevBuf &b, size_t bytes, bool zero) {
+        if (scratch) {
+            work.arena->carve(b, bytes, zero, st);
+            return;
+        }
         if (capture) {
             if (b.bytes < bytes || !b.p) throw Error(GIMIC_ESTATE, "capture before reserve");
             return;
         }
         b.alloc(bytes, st, zero);
     };
+    // mixed kernel: the tuned choice, or in Scratch mode without one the default
+    int mv = c->mixed_variant, msplit = c->mixed_split;
+    if (scratch && prec == GIMIC_PREC_MIXED && mv < 0) {
+        mv = kDefaultVariant;
+        msplit = kDefaultSplit;
+    }
 
     const size_t ld = (size_t)c->lda;
     // chunk of points per pass: basis values + gradients of one chunk stay
-    // within c->chunk_bytes (FP64: 32 bytes per point and function)
-    // (mixed: FP32 values 4 + split values 6 + FP32 gradients 12 bytes)
-    const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 22);
+    // within c->chunk_bytes (FP64: 32 bytes per point and function; mixed:
+    // split values 6 + FP32 gradients 12 bytes)
+    const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 18);
     // row granularity: a multiple of every contraction kernel's workgroup
-    // height (128; 256 for the 256-row pre-split kernel)
+    // height (128; 256 for the 256-row pre-split kernels)
     const size_t gran = prec == GIMIC_PREC_FP64 ? (size_t)kWgRows : 256;
     const size_t rows_all = round_up((size_t)npts, gran);
-    size_t chunk = std::max<size_t>(gran, (c->chunk_bytes / per_row) / gran * gran);
+    size_t chunk = scratch ? work.max_rows
+                           : std::max<size_t>(gran, (c->chunk_bytes / per_row) / gran * gran);
     // as few chunks as the cap allows, all of one size (a short last chunk
     // would be a launch of its own with a tail that fills the GPU badly)
-    const size_t nchunk_min = (rows_all + chunk - 1) / chunk;
-    chunk = round_up((rows_all + nchunk_min - 1) / nchunk_min, gran);
+    auto even_chunks = [&](size_t cap) {
+        const size_t nmin = (rows_all + cap - 1) / cap;
+        return round_up((rows_all + nmin - 1) / nmin, gran);
+    };
+    chunk = even_chunks(chunk);
+    // padding columns and rows of the basis buffers must read as zero: they
+    // are cleared when (re)allocated (every Scratch-mode call carves them
+    // afresh) and never written afterwards.  On an allocation failure in
+    // eager mode the other precision's basis buffers are released first, then
+    // the chunk is halved (smaller chunks change only the tiles' block lists,
+    // i.e. the results' last bits)
+    for (;;) {
+        const size_t rc = chunk;
+        try {
+            if (prec == GIMIC_PREC_FP64) {
+                grow(W.phi64, rc * ld * 8, W.phi64.bytes < rc * ld * 8);
+                grow(W.grad64, 3 * rc * ld * 8, W.grad64.bytes < 3 * rc * ld * 8);
+            } else {
+                grow(W.phis, rc * ld * 6, W.phis.bytes < rc * ld * 6);
+                grow(W.grad32, 3 * rc * ld * 4, W.grad32.bytes < 3 * rc * ld * 4);
+            }
+            break;
+        } catch (const Error &e) {
+            if (e.code != GIMIC_ENOMEM || mode != EvalMode::Eager) throw;
+            (void)hipGetLastError();  // the failed hipMalloc is not a launch error
+            if (prec == GIMIC_PREC_FP64 && (W.phis.p || W.grad32.p)) {
+                W.phis.release();
+                W.grad32.release();
+            } else if (prec != GIMIC_PREC_FP64 && (W.phi64.p || W.grad64.p)) {
+                W.phi64.release();
+                W.grad64.release();
+            } else if (chunk > gran) {
+                chunk = even_chunks(std::max(gran, chunk / 2 / gran * gran));
+            } else {
+                throw;
+            }
+        }
+    }
     const size_t rows_cap = chunk;
     const long long gplane = (long long)(rows_cap * ld);
-    // padding columns and rows of the basis buffers must read as zero: they
-    // are cleared when (re)allocated and never written afterwards
-    if (prec == GIMIC_PREC_FP64) {
-        grow(c->phi64, rows_cap * ld * 8, c->phi64.bytes < rows_cap * ld * 8);
-        grow(c->grad64, 3 * rows_cap * ld * 8, c->grad64.bytes < 3 * rows_cap * ld * 8);
-    } else {
-        grow(c->phi32, rows_cap * ld * 4, c->phi32.bytes < rows_cap * ld * 4);
-        grow(c->phis, rows_cap * ld * 6, c->phis.bytes < rows_cap * ld * 6);
-        grow(c->grad32, 3 * rows_cap * ld * 4, c->grad32.bytes < 3 * rows_cap * ld * 4);
-    }
     // spatial tiles: sorted points, caller indices, sort scratch, and block
     // lists for row tiles of 128 rows or more
     const int nblk = c->lda / 32;
-    grow(c->spts, rows_cap * 3 * 8, false);
-    grow(c->perm, rows_cap * 4, false);
-    grow(c->bpart, 256 * 6 * 8, false);
-    grow(c->sortscr, (size_t)sort_scratch_ints((int)rows_cap) * 4, false);
-    grow(c->bcount, rows_cap / 128 * 4, false);
-    grow(c->blist, rows_cap / 128 * nblk * 4, false);
-    grow(c->toff, (rows_cap / 128 + 1) * 4, false);
-    grow(c->k8count, rows_cap / 128 * 4, false);
-    grow(c->k8list, rows_cap / 128 * nblk * 4 * 4, false);
-    grow(c->bzero, 12 * 1024, c->bzero.bytes < 12 * 1024);  // zeroed once, never written
+    grow(W.spts, rows_cap * 3 * 8, false);
+    grow(W.perm, rows_cap * 4, false);
+    grow(W.bpart, 256 * 6 * 8, false);
+    grow(W.sortscr, (size_t)sort_scratch_ints((int)rows_cap) * 4, false);
+    grow(W.bcount, rows_cap / 128 * 4, false);
+    grow(W.blist, rows_cap / 128 * nblk * 4, false);
+    grow(W.toff, (rows_cap / 128 + 1) * 4, false);
+    grow(W.k8count, rows_cap / 128 * 4, false);
+    grow(W.k8list, rows_cap / 128 * nblk * 4 * 4, false);
+    grow(W.bzero, 12 * 1024, W.bzero.bytes < 12 * 1024);  // zeroed once, never written
     const int nmatset = M.nmatset;
     // chunks in rotated order, the middle one first: the autotuner times the
     // first chunk it sees, and the middle of a grid around a molecule is
@@ -657,7 +796,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         const int n_c = (int)std::min(chunk, (size_t)npts - start);
         const int rows = (int)round_up((size_t)n_c, gran);
 
-        const double *spts = c->spts.as<double>();
+        const double *spts = W.spts.as<double>();
         EvalArgs ea{};
         ea.pts = spts;
         ea.npts = n_c;
@@ -671,11 +810,10 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         ea.turbomole = c->basis.turbomole;
         ea.spherical = c->basis.spherical;
         ea.lda = c->lda;
-        ea.phi64 = c->phi64.as<double>();
-        ea.grad64 = c->grad64.as<double>();
-        ea.phi32 = c->phi32.as<float>();
-        ea.grad32 = c->grad32.as<float>();
-        ea.phis = c->phis.as<__bf16>();
+        ea.phi64 = W.phi64.as<double>();
+        ea.grad64 = W.grad64.as<double>();
+        ea.grad32 = W.grad32.as<float>();
+        ea.phis = W.phis.as<__bf16>();
         ea.lmax = 0;
         for (auto &sh : c->basis.shells) ea.lmax = std::max(ea.lmax, sh.l);
 
@@ -687,22 +825,20 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         ca.nmatset = nmatset;
         ca.pts = spts;
         ca.npts = n_c;
-        ca.a64 = c->phi64.as<double>();
-        ca.g64 = c->grad64.as<double>();
+        ca.a64 = W.phi64.as<double>();
+        ca.g64 = W.grad64.as<double>();
         ca.b64 = M.b64.as<double>();
-        ca.a32 = c->phi32.as<float>();
-        ca.g32 = c->grad32.as<float>();
-        ca.b32 = M.b32.as<float>();
-        ca.as3 = c->phis.as<__bf16>();
+        ca.g32 = W.grad32.as<float>();
+        ca.as3 = W.phis.as<__bf16>();
         ca.bs3 = M.bs3.as<__bf16>();
         ca.rf64 = c->rf64.as<double>();
         ca.rf32 = c->rf32.as<float>();
-        ca.bcount = c->bcount.as<int>();
-        ca.blist = c->blist.as<int>();
+        ca.bcount = W.bcount.as<int>();
+        ca.blist = W.blist.as<int>();
         ca.nblk = nblk;
-        ca.k8count = c->k8count.as<int>();
-        ca.k8list = c->k8list.as<int>();
-        ca.bzero = c->bzero.as<__bf16>();
+        ca.k8count = W.k8count.as<int>();
+        ca.k8list = W.k8list.as<int>();
+        ca.bzero = W.bzero.as<__bf16>();
         // screening block lists for row tiles of tile_rows rows
         auto lists = [&](int tile_rows) {
             TileArgs ta{};
@@ -715,10 +851,10 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             ta.blk_sh = c->blk_sh.as<int>();
             ta.shells = c->shells.as<DevShell>();
             ta.coords = c->coords.as<double>();
-            ta.bcount = c->bcount.as<int>();
-            ta.blist = c->blist.as<int>();
-            ta.k8count = c->k8count.as<int>();
-            ta.k8list = c->k8list.as<int>();
+            ta.bcount = W.bcount.as<int>();
+            ta.blist = W.blist.as<int>();
+            ta.k8count = W.k8count.as<int>();
+            ta.k8list = W.k8list.as<int>();
             hip_check(launch_block_lists(ta, st), "block_lists");
             if (c->tile_stats && mode == EvalMode::Eager) {
                 const int nt = rows / tile_rows;
@@ -750,71 +886,63 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             int nsplit = (int)std::min<long long>(ca.ntiles,
                                                   std::max<long long>(1, (8192 + per - 1) / per));
             if (want_split > 0) nsplit = std::min(want_split, ca.ntiles);
+            if (scratch) nsplit = std::min(nsplit, kScratchMaxSplit);
             ca.tiles_per_split = (ca.ntiles + nsplit - 1) / nsplit;
             ca.nsplit = (ca.ntiles + ca.tiles_per_split - 1) / ca.tiles_per_split;
-            grow(c->part, (size_t)nmatset * ca.nsplit * kNQ * rows * 8, false);
-            ca.part = c->part.as<double>();
+            grow(W.part, (size_t)nmatset * ca.nsplit * kNQ * rows * 8, false);
+            ca.part = W.part.as<double>();
         };
         auto plan_variant = [&](int variant, int split = -1) {
-            if (split < 0) split = c->mixed_split;
-            if (variant == 5 || variant >= 8) plan(256, contract_mx_cols(), split);
-            else if (variant >= 3) plan(contract_mx_rows(), contract_mx_cols(), split);
-            else if (variant == 2) plan(contract_mx_rows(), contract_mx_cols());
-            else plan(kWgRows, kNT);
+            if (split < 0) split = msplit;
+            if (variant == 0) plan(kWgRows, kNT);
+            else plan(mixed_tile_rows(variant), kMx2Cols / kNMat, split);
         };
-        // pre-split 16x16 kernels: row-major partial sums added into a cleared
-        // buffer, and basis rows compressed to each row tile's listed blocks
-        // (lists and offsets built before the basis evaluation)
-        auto presplit = [](int variant) { return (variant >= 3 && variant <= 5) || variant >= 8; };
-        auto tile_rows_of = [&](int variant) {
-            return variant == 5 || variant >= 8 ? 256 : contract_mx_rows();
-        };
+        // mixed kernels: row-major partial sums added into a cleared buffer,
+        // and basis rows compressed to each row tile's listed blocks (lists
+        // and offsets built before the basis evaluation)
         auto run = [&](int variant, int split = -1) {
             plan_variant(variant, split);
-            if (presplit(variant))
+            if (variant != 0)
                 hip_check(launch_fill_zero(ca.part, (size_t)nmatset * ca.nsplit * kNQ * rows, st),
                           "fill_zero");
             if (variant == 0) lists(kWgRows);  // the FP64 kernel walks lists over dense rows
-            ca.toff = presplit(variant) ? c->toff.as<int>() : nullptr;
+            ca.toff = variant != 0 ? W.toff.as<int>() : nullptr;
             if (variant == 0) hip_check(launch_contract_f64(ca, st), "contract_f64");
-            else if (variant == 1) hip_check(launch_contract_bf16x6(ca, st), "contract_bf16x6");
-            else if (variant == 2) hip_check(launch_contract_mx(ca, st), "contract_mx");
             else if (variant == 3) hip_check(launch_contract_mx2(ca, 2, 2, 4, st), "contract_mx2");
             else if (variant == 4) hip_check(launch_contract_mx2(ca, 3, 2, 4, st), "contract_mx2");
             else if (variant == 5) hip_check(launch_contract_mx2(ca, 2, 4, 4, st), "contract_mx2");
             else if (variant == 8) hip_check(launch_contract_mx2(ca, 2, 2, 8, st), "contract_mx2");
-            else if (variant == 9) hip_check(launch_contract_mx2(ca, 3, 2, 8, st), "contract_mx2");
-            else hip_check(launch_contract_mx3(ca, variant == 6 ? 2 : 3, st), "contract_mx3");
+            else hip_check(launch_contract_mx2(ca, 3, 2, 8, st), "contract_mx2");
         };
-        // basis evaluation output each contraction kernel reads
-        auto eval_mode = [](int variant) { return variant == 0 ? 0 : variant >= 3 ? 2 : 1; };
+        // basis evaluation: FP64 rows (mode 0), or the compressed split rows
+        // the mixed kernels read (mode 2)
         auto eval = [&](int variant) {
             ea.toff = nullptr;
-            if (presplit(variant)) {
-                const int tr = tile_rows_of(variant);
+            if (variant != 0) {
+                const int tr = mixed_tile_rows(variant);
                 lists(tr);
-                hip_check(launch_tile_offsets(c->bcount.as<int>(), rows / tr, c->toff.as<int>(), st),
+                hip_check(launch_tile_offsets(W.bcount.as<int>(), rows / tr, W.toff.as<int>(), st),
                           "tile_offsets");
-                ea.bcount = c->bcount.as<int>();
-                ea.blist = c->blist.as<int>();
-                ea.toff = c->toff.as<int>();
+                ea.bcount = W.bcount.as<int>();
+                ea.blist = W.blist.as<int>();
+                ea.toff = W.toff.as<int>();
                 ea.nblk = nblk;
                 ea.tile_rows = tr;
             }
-            hip_check(launch_basis_eval(ea, eval_mode(variant), st), "basis_eval");
+            hip_check(launch_basis_eval(ea, variant == 0 ? 0 : 2, st), "basis_eval");
         };
         // tensor assembly (and flux contributions) of this chunk from the
         // partial sums of the last contraction launch
         auto finalize = [&](int variant) {
             FinalizeArgs fa{};
-            fa.part = c->part.as<double>();
+            fa.part = W.part.as<double>();
             fa.nsplit = ca.nsplit;
-            fa.part_rows = presplit(variant) ? 1 : 0;
+            fa.part_rows = variant != 0 ? 1 : 0;
             fa.rows = rows;
             fa.npts = n_c;
             fa.nmatset = nmatset;
             fa.pts = spts;
-            fa.perm = c->perm.as<int>();
+            fa.perm = W.perm.as<int>();
             fa.giao = c->giao;
             fa.diamag = c->diamag;
             fa.paramag = c->paramag;
@@ -837,27 +965,26 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             }
             hip_check(launch_finalize(fa, st), "finalize");
         };
-        const bool tune = prec == GIMIC_PREC_MIXED && c->mixed_variant < 0;
+        const bool tune = prec == GIMIC_PREC_MIXED && mv < 0;
         if (capture && tune) throw Error(GIMIC_ESTATE, "mixed kernel not tuned before capture");
         if (mode == EvalMode::Reserve && !tune) {
-            plan_variant(prec == GIMIC_PREC_FP64 ? 0 : c->mixed_variant);
+            plan_variant(prec == GIMIC_PREC_FP64 ? 0 : mv);
             continue;
         }
         // rows in Morton order of their cells (spts, perm)
         timed(c, 0, st, [&] {
-            hip_check(launch_sort_points(d_r + 3 * start, n_c, c->spts.as<double>(),
-                                         c->perm.as<int>(), c->bpart.as<double>(),
-                                         c->sortscr.as<int>(), st),
+            hip_check(launch_sort_points(d_r + 3 * start, n_c, W.spts.as<double>(),
+                                         W.perm.as<int>(), W.bpart.as<double>(),
+                                         W.sortscr.as<int>(), st),
                       "sort_points");
         });
         if (tune) {
             // autotune once per context on this chunk: basis evaluation +
-            // contraction for the pre-split kernels that walk block lists,
-            // each with several N-split counts (more splits give more
-            // workgroups that share each row tile's A through L2, fewer
-            // give each workgroup more N-blocks to amortise its start);
-            // every candidate is timed twice and its faster run counts; keep
-            // the fastest
+            // contraction for the mixed kernels, each with several N-split
+            // counts (more splits give more workgroups that share each row
+            // tile's A through L2, fewer give each workgroup more N-blocks to
+            // amortise its start); every candidate is timed twice and its
+            // faster run counts; keep the fastest
             float best = 0.f;
             int best_split = 0;
             const int cand[][2] = {{3, 4}, {4, 4}, {5, 4}, {8, 2}, {8, 4}, {8, 8},
@@ -891,12 +1018,14 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
                 }
             }
             c->mixed_split = best_split;
+            mv = c->mixed_variant;
+            msplit = c->mixed_split;
             if (mode == EvalMode::Reserve) {
-                plan_variant(c->mixed_variant);
+                plan_variant(mv);
                 continue;
             }
         }
-        const int variant = prec == GIMIC_PREC_FP64 ? 0 : c->mixed_variant;
+        const int variant = prec == GIMIC_PREC_FP64 ? 0 : mv;
         timed(c, 0, st, [&] { eval(variant); });
         timed(c, 1, st, [&] { run(variant); });
         timed(c, 2, st, [&] { finalize(variant); });
@@ -909,28 +1038,48 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
 // nucleus, where J is large and FP32 products lose the most relative
 // accuracy, are selected on the device (in index order), evaluated again in
 // FP64, and their tensors and flux contributions overwrite the mixed ones in
-// place.  Synchronises once, to size the subset.
+// place.  Synchronises once, to size the subset.  With an arena in `work`,
+// its buffers and those of the FP64 evaluation are carved from the arena's
+// free part.
 void refine_near_nuclei(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
-                        const std::vector<int> &spins, hipStream_t st, const FluxDev *flux) {
+                        const std::vector<int> &spins, hipStream_t st, const FluxDev *flux,
+                        EvalWork work) {
     c->refined_last = 0;
     if (!(c->fp64_radius > 0.0) || npts <= 0) return;
     if (npts > 0x7fffffffLL) throw Error(GIMIC_EINVAL, "too many points for the FP64 pass");
-    c->near_idx.alloc((size_t)npts * 8, st, false);
-    c->near_scr.alloc(near_select_scratch(npts) * 4, st, false);
-    c->near_cnt.alloc(8, st, false);
+    Workspace &W = work.ws ? *work.ws : c->ws;
+    auto get = [&](DevBuf &b, size_t bytes) {
+        if (work.arena) work.arena->carve(b, bytes, false, st);
+        else b.alloc(bytes, st, false);
+    };
+    get(W.near_idx, (size_t)npts * 8);
+    get(W.near_scr, near_select_scratch(npts) * 4);
+    get(W.near_cnt, 8);
     hip_check(launch_near_select(d_r, npts, c->coords.as<double>(), c->basis.natoms(),
-                                 c->fp64_radius, c->near_idx.as<long long>(),
-                                 c->near_cnt.as<long long>(), c->near_scr.as<int>(), st),
+                                 c->fp64_radius, W.near_idx.as<long long>(),
+                                 W.near_cnt.as<long long>(), W.near_scr.as<int>(), st),
               "near_select");
     long long k = 0;
-    hip_check(hipMemcpyAsync(&k, c->near_cnt.p, 8, hipMemcpyDeviceToHost, st), "copy count");
+    hip_check(hipMemcpyAsync(&k, W.near_cnt.p, 8, hipMemcpyDeviceToHost, st), "copy count");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     if (k <= 0) return;
-    c->near_pts.alloc((size_t)k * 24, st, false);
-    hip_check(launch_gather_points(d_r, c->near_idx.as<long long>(), k, c->near_pts.as<double>(), st),
+    get(W.near_pts, (size_t)k * 24);
+    hip_check(launch_gather_points(d_r, W.near_idx.as<long long>(), k, W.near_pts.as<double>(), st),
               "gather_points");
-    evaluate(c, k, c->near_pts.as<double>(), d_jt, spins, GIMIC_PREC_FP64, st, flux,
-             EvalMode::Eager, c->near_idx.as<long long>(), npts);
+    if (work.arena) {
+        // the FP64 evaluation carves its own views after the selection's
+        Workspace sub;
+        Arena &A = *work.arena;
+        const size_t used = round_up(A.used, 256);
+        const size_t rows = scratch_rows(c, GIMIC_PREC_FP64, (int)spins.size(),
+                                         A.size > used ? A.size - used : 0, kScratchMaxSplit);
+        if (rows == 0) throw Error(GIMIC_ENOMEM, "device scratch too small for the FP64 pass");
+        evaluate(c, k, W.near_pts.as<double>(), d_jt, spins, GIMIC_PREC_FP64, st, flux,
+                 EvalMode::Scratch, W.near_idx.as<long long>(), npts, EvalWork{&sub, &A, rows});
+    } else {
+        evaluate(c, k, W.near_pts.as<double>(), d_jt, spins, GIMIC_PREC_FP64, st, flux,
+                 EvalMode::Eager, W.near_idx.as<long long>(), npts);
+    }
     c->refined_last = k;
 }
 
@@ -1195,7 +1344,9 @@ void flux_run(gimic_ctx *c, int64_t npts, int nplanes, const int64_t *offs, cons
         reduce();
     } else {
         // the inputs' preparation, all planes, every chunk and the per-plane
-        // reductions as one graph
+        // reductions as one graph.  An untuned mixed kernel is tuned in the
+        // Reserve pass on the points themselves, so they are prepared first
+        if (precision == GIMIC_PREC_MIXED && c->mixed_variant < 0) prepare(st);
         evaluate(c, npts, d_r, c->jt_tmp.as<double>(), {spin}, precision, st, &fd,
                  EvalMode::Reserve);
         auto &M = get_mats(c, {spin}, precision);
@@ -1206,13 +1357,13 @@ void flux_run(gimic_ctx *c, int64_t npts, int nplanes, const int64_t *offs, cons
             (uintptr_t)c->mixed_variant, (uintptr_t)st, (uintptr_t)c->pts_tmp.p,
             (uintptr_t)c->jt_tmp.p, (uintptr_t)c->contrib.p, (uintptr_t)c->planes.p,
             (uintptr_t)c->pparams.p, (uintptr_t)c->offs.p, (uintptr_t)c->outs.p,
-            (uintptr_t)c->phi64.p, (uintptr_t)c->grad64.p, (uintptr_t)c->phi32.p,
-            (uintptr_t)c->grad32.p, (uintptr_t)c->part.p, (uintptr_t)M.b64.p,
-            (uintptr_t)M.b32.p, (uintptr_t)c->phis.p, (uintptr_t)M.bs3.p,
-            (uintptr_t)c->mixed_split, (uintptr_t)c->spts.p, (uintptr_t)c->perm.p,
-            (uintptr_t)c->bpart.p, (uintptr_t)c->sortscr.p, (uintptr_t)c->bcount.p,
-            (uintptr_t)c->blist.p, (uintptr_t)c->toff.p, (uintptr_t)c->k8count.p,
-            (uintptr_t)c->k8list.p, (uintptr_t)c->bzero.p, (uintptr_t)refine,
+            (uintptr_t)c->ws.phi64.p, (uintptr_t)c->ws.grad64.p, (uintptr_t)c->ws.grad32.p,
+            (uintptr_t)c->ws.part.p, (uintptr_t)M.b64.p, (uintptr_t)c->ws.phis.p,
+            (uintptr_t)M.bs3.p, (uintptr_t)c->mixed_split, (uintptr_t)c->ws.spts.p,
+            (uintptr_t)c->ws.perm.p, (uintptr_t)c->ws.bpart.p, (uintptr_t)c->ws.sortscr.p,
+            (uintptr_t)c->ws.bcount.p, (uintptr_t)c->ws.blist.p, (uintptr_t)c->ws.toff.p,
+            (uintptr_t)c->ws.k8count.p, (uintptr_t)c->ws.k8list.p, (uintptr_t)c->ws.bzero.p,
+            (uintptr_t)refine,
             (uintptr_t)c->chunk_bytes};
         key.insert(key.end(), source.begin(), source.end());
         if (!c->graph_exec || key != c->graph_key) {
@@ -1387,7 +1538,9 @@ int gimic_ctx_tuning_log(const gimic_ctx *c, int *n, int *codes, double *ms) {
 int gimic_ctx_set_mixed_kernel(gimic_ctx *c, int variant, int split) {
     return guarded([&] {
         if (!c) throw Error(GIMIC_EINVAL, "null context");
-        if (variant < 0 || variant > 9) throw Error(GIMIC_EINVAL, "mixed kernel must be 0..9");
+        if (variant != 0 && variant != 3 && variant != 4 && variant != 5 && variant != 8 &&
+            variant != 9)
+            throw Error(GIMIC_EINVAL, "mixed kernel must be 0 (tune), 3, 4, 5, 8 or 9");
         if (split < 0) throw Error(GIMIC_EINVAL, "split count must be >= 0");
         c->mixed_variant = variant == 0 ? -1 : variant;
         c->mixed_split = variant >= 3 ? split : 0;
@@ -1407,8 +1560,10 @@ int gimic_ctx_set_fp64_radius(gimic_ctx *c, double radius) {
 int gimic_ctx_set_chunk_bytes(gimic_ctx *c, int64_t bytes) {
     return guarded([&] {
         if (!c) throw Error(GIMIC_EINVAL, "null context");
-        if (bytes < 0) throw Error(GIMIC_EINVAL, "chunk bytes must be >= 0");
-        c->chunk_bytes = bytes == 0 ? c->chunk_bytes_default : (size_t)bytes;
+        if (bytes < -1) throw Error(GIMIC_EINVAL, "chunk bytes must be >= -1");
+        c->chunk_bytes = bytes == 0    ? kChunkBytesDefault
+                         : bytes == -1 ? c->chunk_bytes_large
+                                       : (size_t)bytes;
     });
 }
 

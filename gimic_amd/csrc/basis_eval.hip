@@ -10,11 +10,12 @@
 //   d_m phi = k_m r_m^(k_m-1)(...) g - 2 r_m x^a y^b z^c h (caos.f90:39-64)
 // with contractions outside their screening radius set to zero
 // (basis.f90:118-136).  Spherical contractions are T_l times the cartesian
-// ones.  Outputs: phi (the contraction's left operand) and its gradient, in
-// FP64 for the FP64 path or FP32 for the mixed path.
+// ones.  Outputs: phi (the contraction's left operand) and its gradient in
+// FP64 for the FP64 path.
 //
-// The split output of the pre-split mixed kernel (phi as three BF16 parts
-// plus FP32 gradients) is produced by basis_eval_staged_kernel: a workgroup
+// The split output of the mixed kernels (phi as three BF16 parts plus FP32
+// gradients), and the FP64 rows of bases up to 4,800 functions, are
+// produced by basis_eval_staged_kernel: a workgroup
 // owns a tile of 1-8 whole point rows, its threads evaluate (point,
 // contraction) pairs into a zeroed LDS image of those rows, and the rows
 // then leave with 16-byte coalesced stores.  Written directly, every store
@@ -126,6 +127,8 @@ __host__ __device__ constexpr int sph_off(int L) {
     return L == 2 ? 0 : L == 3 ? 30 : L == 4 ? 100 : L == 5 ? 235 : 0;
 }
 
+// MODE 0: FP64 rows in global memory; 3: split values and FP32 gradients
+// into the staged kernel's LDS image; 4: FP64 values and gradients into it
 template <int MODE>
 __device__ __forceinline__ void store(const EvalArgs &a, size_t row, int f, double v,
                                       double gx, double gy, double gz) {
@@ -136,11 +139,6 @@ __device__ __forceinline__ void store(const EvalArgs &a, size_t row, int f, doub
         a.grad64[i] = gx;
         a.grad64[plane + i] = gy;
         a.grad64[2 * plane + i] = gz;
-    } else if (MODE == 1) {
-        a.phi32[i] = (float)v;
-        a.grad32[i] = (float)gx;
-        a.grad32[plane + i] = (float)gy;
-        a.grad32[2 * plane + i] = (float)gz;
     } else if (MODE == 3) {
         // into the LDS tile: `row` is the point's row within the tile
         const int tp = a.tile_points, lda = a.lda;
@@ -165,20 +163,6 @@ __device__ __forceinline__ void store(const EvalArgs &a, size_t row, int f, doub
         d[pl] = gx;
         d[2 * pl] = gy;
         d[3 * pl] = gz;
-    } else {
-        // value as three BF16 parts, x = h + m + l exactly
-        const float x = (float)v;
-        const __bf16 hi = (__bf16)x;
-        const float r1 = x - (float)hi;
-        const __bf16 mi = (__bf16)r1;
-        const __bf16 lo = (__bf16)(r1 - (float)mi);
-        __bf16 *d = a.phis + row * 3 * (size_t)a.lda + (f >> 3) * 24 + (f & 7);
-        d[0] = hi;
-        d[8] = mi;
-        d[16] = lo;
-        a.grad32[i] = (float)gx;
-        a.grad32[plane + i] = (float)gy;
-        a.grad32[2 * plane + i] = (float)gz;
     }
 }
 
@@ -434,8 +418,6 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
     }
     if (mode == 0)
         hipLaunchKernelGGL(basis_eval_kernel<0>, dim3(nblk), dim3(256), 0, s, a);
-    else if (mode == 1)
-        hipLaunchKernelGGL(basis_eval_kernel<1>, dim3(nblk), dim3(256), 0, s, a);
     else {
         // dense: the largest tile of 1, 2, 4 or 8 rows whose LDS image fits in
         // 72 KiB (two workgroups per CU).  Compressed: 8 rows per workgroup

@@ -90,10 +90,10 @@ struct EvalArgs {
     int lda;             // padded K (multiple of kKPad)
     // FP64 outputs
     double *phi64, *grad64;  // [rows][lda], [3][rows][lda]
-    // mixed outputs (FP32 values; the contraction splits them into BF16)
-    float *phi32, *grad32;   // [rows][lda], [3][rows][lda]
-    // split basis values for the pre-split contraction: each FP32 value x as
-    // three BF16 parts x = h + m + l (exact), layout [row][lda/8][3][8]
+    // mixed outputs: FP32 gradients, and each value x as three BF16 parts
+    // x = h + m + l (exact), layout [row][lda/8][3][8] (compressed to the
+    // listed blocks of each row tile, below)
+    float *grad32;
     __bf16 *phis;
     int tile_points;     // points per workgroup of the LDS-staged kernel (set by the launcher)
     size_t lds_image;    // bytes of its LDS image (set by the launcher)
@@ -116,8 +116,8 @@ struct ContractArgs {
     const double *pts;   // chunk points (3 per row, npts valid rows)
     int npts;
     const double *a64, *g64, *b64;  // FP64 path
-    const float *a32, *g32, *b32;   // mixed path (FP32 operands, split to BF16 x3)
-    const __bf16 *as3;   // split basis values [rows][lda/8][3][8] (pre-split kernel)
+    const float *g32;    // mixed path: FP32 gradients
+    const __bf16 *as3;   // split basis values [rows][lda/8][3][8] (mixed kernels)
     const __bf16 *bs3;   // split right operand, LDS-image blocks per (set, N-tile, K-step)
     const double *rf64;  // [3][ncol_pad] atom coordinate of each function
     const float *rf32;
@@ -135,10 +135,6 @@ struct ContractArgs {
     const int *toff;
 };
 hipError_t launch_contract_f64(const ContractArgs &a, hipStream_t s);
-hipError_t launch_contract_bf16x6(const ContractArgs &a, hipStream_t s);
-hipError_t launch_contract_mx(const ContractArgs &a, hipStream_t s);
-int contract_mx_rows();  // row tile of launch_contract_mx (a multiple of kWgRows)
-int contract_mx_cols();  // functions per N-tile of launch_contract_mx
 // Pre-split variant: both operands arrive as BF16 parts, the right operand in
 // blocks that are byte images of the kernel's LDS buffer (see contract_mx2.hip)
 // nbuf 2 or 3 LDS buffers, krb 2 or 4 row blocks per wavefront, 4 or 8
@@ -146,8 +142,6 @@ int contract_mx_cols();  // functions per N-tile of launch_contract_mx
 // 2 or 3, or krb 4, nbuf 2; 8 waves with krb 2, nbuf 2 or 3).  Adds its
 // partial sums to a.part, row-major, which must be zero before the launch.
 hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, int waves, hipStream_t s);
-// Pre-split operands on 32x32x16 MFMA blocks, 128 rows per workgroup (contract_mx3.hip)
-hipError_t launch_contract_mx3(const ContractArgs &a, int nbuf, hipStream_t s);
 constexpr int kMx2Cols = 224;                         // 7 matrices x 32 functions
 constexpr int kMx2BlockElems = 22528;                 // LDS buffer: 3*4*224*8 = 21504, padded to 44 KiB
 // pre-split right operand in HBM: per (density set, N-tile, 8-function group

@@ -43,12 +43,6 @@ def integrate_planes(ctx, grids, bfields, spin="total", precision="fp64"):
                            precision=precision)
 
 
-def integrate_plane_grids(ctx, grids, bfields, spin="total", precision="fp64", modulus=False):
-    """as integrate_planes, with the grid points generated on the device from
-    each grid's 1-D quadrature rules (product grids only)"""
-    return ctx.flux_plane_grids(grids, bfields, spin=spin, precision=precision, modulus=modulus)
-
-
 def report(net, pos, neg):
     """the reference's block (integral.f90:167-183)"""
     lines = ["*" * 60,

@@ -110,6 +110,7 @@ def load():
         fn = getattr(L, name)
         fn.argtypes = args
         if name.startswith(("gimic_ctx", "gimic_mlp")) or name in ("gimic_jtensor_batch", "gimic_jtensor_batch_device",
+                                                   
                                                     "gimic_jtensor_multi_device", "gimic_flux_planes",
                                                     "gimic_flux_plane_grids",
                                                     "gimic_sph_transform", "gimic_quadrature",
@@ -284,13 +285,10 @@ class Context:
         return [(int(c) // 100, int(c) % 100, float(t)) for c, t in zip(codes[:n.value], ms)]
 
     def set_mixed_kernel(self, variant=0, split=0):
-        """fix the mixed contraction kernel (1 register-tiled, 2 LDS-tiled,
-        3/4 pre-split double/triple buffered, 5 pre-split with 256-row
-        tiles, 6/7 pre-split on 32x32 MFMA blocks double buffered/pipelined,
-        8/9 pre-split with 8 wavefronts and 256-row tiles double/triple
-        buffered; 3-5, 8 and 9 visit only the blocks screening leaves non-zero,
-        with `split` N-ranges, 0 = by workgroup count) or 0 for automatic
-        selection"""
+        """fix the mixed contraction kernel (3/4: 4 wavefronts, 128-row tiles,
+        double/triple buffered; 5: 4 wavefronts, 256-row tiles; 8/9: 8
+        wavefronts, 256-row tiles, double/triple buffered; `split` N-ranges
+        per row tile, 0 = by workgroup count) or 0 for automatic selection"""
         check(self._lib.gimic_ctx_set_mixed_kernel(self._h, int(variant), int(split)))
 
     def set_fp64_radius(self, radius):
@@ -300,7 +298,7 @@ class Context:
 
     def set_chunk_bytes(self, nbytes):
         """device memory for the basis buffers of one chunk of points (0: the
-        default, a third of the device's memory and at most 96 GiB)"""
+        default, 8 GiB; -1: a third of the device's memory, at most 96 GiB)"""
         check(self._lib.gimic_ctx_set_chunk_bytes(self._h, int(nbytes)))
 
     def chunk_bytes(self):

@@ -249,8 +249,18 @@ class Surrogate:
         self.cell_ok = (self._cell_err <= tol) & (self._cell_cnt >= self.min_holdout)
         self.stats["cells_served"] = int(self.cell_ok.sum())
 
+    def inside(self, r):
+        """points in the closed training box (cell_index clamps the others
+        into the edge cells, where the network was never checked)"""
+        import torch
+        lo = torch.as_tensor(self.lo, device=r.device)
+        hi = torch.as_tensor(self.hi, device=r.device)
+        return ((r >= lo) & (r <= hi)).all(dim=1)
+
     def served_mask(self, r):
-        return self.cell_ok[self.cell_index(r)] & ~self.near_nucleus(r)
+        """the network serves a point only inside the box, in a cell whose
+        held-out error met the tolerance, and beyond the exclusion radius"""
+        return self.cell_ok[self.cell_index(r)] & self.inside(r) & ~self.near_nucleus(r)
 
     def evaluate(self, r, out=None):
         """J at every point of r: network where served, exact elsewhere;

@@ -130,25 +130,23 @@ int gimic_ctx_set_timing(gimic_ctx *ctx, int on);
 int gimic_ctx_get_timing(gimic_ctx *ctx, double *ms, long long *launches);
 
 /* Mixed-precision kernel selection (no counterpart in the reference).
- * Kernels: register-tiled (1), LDS-tiled (2), pre-split operands on 16x16
- * MFMA blocks double (3) or triple (4) buffered with 128 rows per workgroup
- * or double buffered with 256 rows (5), pre-split on 32x32 MFMA blocks
- * double buffered (6) or pipelined (7), pre-split on 16x16 MFMA blocks with
- * 8 wavefronts and 256 rows per workgroup, double (8) or triple (9)
- * buffered.  Kernels 3 to 5, 8 and 9 (and the FP64 kernel) visit only the
- * 32-function blocks that screening leaves non-zero over their row tile;
- * 1, 2, 6 and 7 compute the dense product.  The first mixed evaluation of a
- * context times basis evaluation + contraction on its middle chunk for
- * kernels 3, 4 and 5 (N-split count 4) and 8 and 9 (N-split counts 2, 4
- * and 8), each twice, and keeps the fastest.  Returns the choice (variant
- * -1 before the first mixed call), its split count (0: chosen by workgroup
- * count) and the best time in ms of each variant (ms[9], 0 for variants not
- * timed). */
+ * Kernels (both operands pre-split into BF16 parts, 16x16 MFMA blocks):
+ * 4 wavefronts with 128 rows per workgroup, double (3) or triple (4)
+ * buffered, 4 wavefronts with 256 rows, double buffered (5), 8 wavefronts
+ * with 256 rows, double (8) or triple (9) buffered.  All of them, and the
+ * FP64 kernel, visit only the 32-function blocks that screening leaves
+ * non-zero over their row tile.  The first mixed evaluation of a context
+ * times basis evaluation + contraction on its middle chunk for kernels 3,
+ * 4 and 5 (N-split count 4) and 8 and 9 (N-split counts 2, 4 and 8), each
+ * twice, and keeps the fastest.  Returns the choice (variant -1 before the
+ * first mixed call), its split count (0: chosen by workgroup count) and the
+ * best time in ms of each variant (ms[9], index variant - 1; 0 for variants
+ * not timed). */
 int gimic_ctx_tuning(const gimic_ctx *ctx, int *variant, int *split, double *ms);
 /* Every candidate the tuner timed: codes[k] = variant * 100 + split count,
  * ms[k] its best time.  *n: capacity in, number of candidates out. */
 int gimic_ctx_tuning_log(const gimic_ctx *ctx, int *n, int *codes, double *ms);
-/* Fix the mixed-precision kernel (1 to 9; for 3 to 9 also the N-split count,
+/* Fix the mixed-precision kernel (3, 4, 5, 8 or 9, with its N-split count,
  * 0 = by workgroup count) or return to automatic selection (variant 0: the
  * next mixed evaluation tunes again). */
 int gimic_ctx_set_mixed_kernel(gimic_ctx *ctx, int variant, int split);
@@ -169,15 +167,17 @@ int gimic_ctx_fp64_refined(const gimic_ctx *ctx, double *radius, int64_t *npts);
 /* Device memory for the basis values and gradients of one chunk of points
  * (no counterpart in the reference, which evaluates point by point).  A call
  * evaluates its points in chunks of equal size whose basis buffers fit in
- * `bytes` (mixed: 22 bytes per point and padded function; FP64: 32), and at
- * least 256 points; 0 restores the default, a third of the device's total
- * memory and at most 96 GiB (one launch for a 128^3 grid with 1800
- * functions).  Each launch ends in a tail that leaves the GPU partly idle,
- * so fewer, larger chunks are faster.  Buffers a context has allocated are
- * kept until it is destroyed: set this before the first evaluation to bound
- * its memory.  Results of different chunk sizes agree to the rounding of
- * the precision (the tiles' block lists depend on the chunk's points).
- * gimic_ctx_chunk_bytes reports the setting in force. */
+ * `bytes` (mixed: 18 bytes per point and padded function; FP64: 32), and at
+ * least 256 points.  0 restores the default, 8 GiB; -1 selects the large
+ * setting, a third of the device's total memory and at most 96 GiB (one
+ * launch for a 128^3 grid with 1800 functions).  Each launch ends in a tail
+ * that leaves the GPU partly idle, so fewer, larger chunks are faster.
+ * Buffers a context has allocated are kept until it is destroyed: set this
+ * before the first evaluation to bound its memory.  When an allocation
+ * fails, an eager call first frees the context's basis buffers of the other
+ * precision, then halves its chunk.  Results of different chunk sizes agree
+ * to the rounding of the precision (the tiles' block lists depend on the
+ * chunk's points).  gimic_ctx_chunk_bytes reports the setting in force. */
 int gimic_ctx_set_chunk_bytes(gimic_ctx *ctx, int64_t bytes);
 int gimic_ctx_chunk_bytes(const gimic_ctx *ctx, int64_t *bytes);
 

@@ -111,7 +111,15 @@ def basis_transform(shell_l):
 
 
 def to_cartesian(dens, shell_l):
-    """(nset,4,nsph,nsph) spherical family -> cartesian D_cart = T^T D T"""
-    T = basis_transform(shell_l)
-    d = np.asarray(dens)
-    return np.einsum("ai,skab,bj->skij", T, d, T, optimize=True)
+    """(nset,4,nsph,nsph) spherical family -> cartesian D_cart = T^T D T
+    (T is block-diagonal by shell: the products skip its exact zeros)"""
+    from scipy import sparse
+    Ts = sparse.csr_matrix(basis_transform(shell_l))
+    d = np.asarray(dens, dtype=np.float64)
+    nc = Ts.shape[1]
+    out = np.empty(d.shape[:2] + (nc, nc))
+    for s in range(d.shape[0]):
+        for k in range(d.shape[1]):
+            x = np.asarray(Ts.T @ d[s, k])          # T^T D        (nc, nsph)
+            out[s, k] = np.asarray(Ts.T @ x.T).T    # (T^T (T^T D)^T)^T = T^T D T
+    return out

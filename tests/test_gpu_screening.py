@@ -49,29 +49,52 @@ def test_result_does_not_depend_on_point_order(cage, precision):
 
 
 def test_chunks_change_launches_not_results(cage):
-    """a chunk limit that splits the points into four or five launches gives
-    the tensors of one launch, to the rounding of each precision; 0 restores
-    the default and a negative size is refused"""
+    """a chunk limit that splits the 24^3 points into four mixed launches
+    (4096 rows of 1824 padded functions at 18 bytes) and six FP64 launches
+    (2304 rows at 32 bytes) gives the tensors of one launch, to the rounding
+    of each precision; 0 restores the default (8 GiB, one launch each), -1
+    selects the large setting and smaller sizes are refused"""
     s, ctx = cage
     pts = _grid(24, 0.15, [3.0, -1.8, -1.8])
-    default = ctx.chunk_bytes()
-    assert default >= 1 << 30
+    assert ctx.chunk_bytes() == 8 << 30
     print("case cage60 chunk size", flush=True)
-    ref = {p: ctx.jtensor(pts, precision=p) for p in ("fp64", "mixed")}
-    small = 1824 * 22 * 4096  # 4096 mixed rows of 1824 padded functions
+    radius = ctx.fp64_refined()[0]
+
+    def launches(precision):
+        # contraction launches of one call (the FP64 pass near nuclei off,
+        # so only this precision's kernel is counted)
+        ctx.set_fp64_radius(0.0)
+        ctx.set_timing(True)
+        try:
+            J = ctx.jtensor(pts, precision=precision)
+            n = ctx.get_timing()["contract"][1]
+        finally:
+            ctx.set_timing(False)
+            ctx.set_fp64_radius(radius)
+        return J, n
+
+    ref = {}
+    for p in ("fp64", "mixed"):
+        ref[p], n = launches(p)
+        assert n == 1, (p, n)
+    small = 1824 * 18 * 4096
     try:
         ctx.set_chunk_bytes(small)
         assert ctx.chunk_bytes() == small
-        j64 = ctx.jtensor(pts, precision="fp64")
-        jm = ctx.jtensor(pts, precision="mixed")
+        j64, n64 = launches("fp64")
+        jm, nm = launches("mixed")
     finally:
         ctx.set_chunk_bytes(0)
-    assert ctx.chunk_bytes() == default
+    assert (n64, nm) == (6, 4)
+    assert ctx.chunk_bytes() == 8 << 30
     assert rel_err(j64, ref["fp64"]).max() <= 1e-12
     assert rel_err(jm, ref["mixed"]).max() <= 1e-5
     assert rel_err(jm, ref["fp64"]).max() <= 1e-5
+    ctx.set_chunk_bytes(-1)
+    assert ctx.chunk_bytes() >= 1 << 30
+    ctx.set_chunk_bytes(0)
     with pytest.raises(RuntimeError):
-        ctx.set_chunk_bytes(-1)
+        ctx.set_chunk_bytes(-2)
 
 
 def test_lists_leave_out_blocks_and_fp64_matches_oracle(cage):
