@@ -527,10 +527,11 @@ gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec)
                   "hipMemcpyAsync");
     } else {
         std::vector<float> b32(B.begin(), B.end());
-        // the FP32 values as three BF16 parts (x = h + m + l exactly),
-        // one 8-function group per (set, N-tile, group kg), [part][column][8 k],
-        // column = matrix * 32 + function within the N-tile; the pre-split
-        // kernels gather four groups per K-step into their LDS buffers
+        // the FP32 values as their first kBParts BF16 parts (three parts:
+        // x = h + m + l exactly), one 8-function group per (set, N-tile,
+        // group kg), [part][column][8], column = matrix * 32 + function
+        // within the N-tile; the mixed kernels gather four groups per K-step
+        // into their LDS buffers
         const size_t nks = ld / 32, ntl = nc / 32, grp = kMx2GroupElems;
         std::vector<uint16_t> bs(spins.size() * ntl * nks * kMx2StepElems + 1024, 0);
         auto bf = [](float x) {
@@ -554,15 +555,13 @@ gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec)
                     for (size_t kg = 0; kg < 4 * nks; kg++) {
                         uint16_t *dst = &bs[((si * ntl + nt) * 4 * nks + kg) * grp];
                         for (int e = 0; e < 8; e++) {
-                            const float x = src[kg * 8 + e];
-                            const uint16_t h = bf(x);
-                            const float r1 = x - fl(h);
-                            const uint16_t mm = bf(r1);
-                            const uint16_t l = bf(r1 - fl(mm));
+                            float x = src[kg * 8 + e];
                             const size_t o = (size_t)col * 8 + e;
-                            dst[o] = h;
-                            dst[part + o] = mm;
-                            dst[2 * part + o] = l;
+                            for (int p = 0; p < kBParts; p++) {
+                                const uint16_t h = bf(x);
+                                dst[p * part + o] = h;
+                                x -= fl(h);
+                            }
                         }
                     }
                 }
@@ -636,7 +635,7 @@ size_t scratch_eval_bytes(const gimic_ctx *c, int prec, int nmatset, size_t rows
     const size_t ld = (size_t)c->lda, nblk = ld / 32, t = rows / 128;
     auto r = [](size_t b) { return round_up(std::max<size_t>(b, 1), 256); };
     size_t n = prec == GIMIC_PREC_FP64 ? r(rows * ld * 8) + r(3 * rows * ld * 8)
-                                       : r(rows * ld * 6) + r(3 * rows * ld * 4);
+                                       : r(rows * ld * 2 * kAParts) + r(3 * rows * ld * 4);
     n += r(rows * 24) + r(rows * 4) + r(256 * 6 * 8) + r((size_t)sort_scratch_ints((int)rows) * 4);
     n += r(t * 4) + r(t * nblk * 4) + r((t + 1) * 4) + r(t * 4) + r(t * nblk * 16) + r(12 * 1024);
     n += r((size_t)nmatset * split * kNQ * rows * 8);
@@ -722,8 +721,8 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
     const size_t ld = (size_t)c->lda;
     // chunk of points per pass: basis values + gradients of one chunk stay
     // within c->chunk_bytes (FP64: 32 bytes per point and function; mixed:
-    // split values 6 + FP32 gradients 12 bytes)
-    const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 18);
+    // split values 2 kAParts + FP32 gradients 12 bytes)
+    const size_t per_row = ld * (prec == GIMIC_PREC_FP64 ? 32 : 2 * kAParts + 12);
     // row granularity: a multiple of every contraction kernel's workgroup
     // height (128; 256 for the 256-row pre-split kernels)
     const size_t gran = prec == GIMIC_PREC_FP64 ? (size_t)kWgRows : 256;
@@ -750,7 +749,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
                 grow(W.phi64, rc * ld * 8, W.phi64.bytes < rc * ld * 8);
                 grow(W.grad64, 3 * rc * ld * 8, W.grad64.bytes < 3 * rc * ld * 8);
             } else {
-                grow(W.phis, rc * ld * 6, W.phis.bytes < rc * ld * 6);
+                grow(W.phis, rc * ld * 2 * kAParts, W.phis.bytes < rc * ld * 2 * kAParts);
                 grow(W.grad32, 3 * rc * ld * 4, W.grad32.bytes < 3 * rc * ld * 4);
             }
             break;
@@ -1696,7 +1695,7 @@ int gimic_quadrature(double a, double b, int order, int npts, int kind, double *
         if (!pts || !wgts || npts < 1 || order < 1 || (kind != 0 && kind != 1))
             throw Error(GIMIC_EINVAL, "bad argument");
         if (npts > 1 && npts % order != 0)
-            throw Error(GIMIC_EINVAL, "npts is not dividable by the quadrature order");
+            throw Error(GIMIC_EINVAL, "the point count must be a multiple of the quadrature order");
         setup_gauss_data(a, b, order, npts, kind == 1, pts, wgts);
     });
 }

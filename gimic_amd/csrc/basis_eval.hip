@@ -140,18 +140,19 @@ __device__ __forceinline__ void store(const EvalArgs &a, size_t row, int f, doub
         a.grad64[plane + i] = gy;
         a.grad64[2 * plane + i] = gz;
     } else if (MODE == 3) {
-        // into the LDS tile: `row` is the point's row within the tile
+        // into the LDS tile: `row` is the point's row within the tile; the
+        // value as its first kAParts BF16 parts (h + m + l = x exactly)
         const int tp = a.tile_points, lda = a.lda;
-        const float x = (float)v;
-        const __bf16 hi = (__bf16)x;
-        const float r1 = x - (float)hi;
-        const __bf16 mi = (__bf16)r1;
-        const __bf16 lo = (__bf16)(r1 - (float)mi);
-        __bf16 *d = (__bf16 *)basis_tile + row * 3 * (size_t)lda + (f >> 3) * 24 + (f & 7);
-        d[0] = hi;
-        d[8] = mi;
-        d[16] = lo;
-        float *gr = (float *)(basis_tile + (size_t)tp * lda * 6) + row * (size_t)lda + f;
+        float x = (float)v;
+        __bf16 *d = (__bf16 *)basis_tile + row * kAParts * (size_t)lda + (f >> 3) * (8 * kAParts) +
+                    (f & 7);
+#pragma unroll
+        for (int k = 0; k < kAParts; k++) {
+            const __bf16 part = (__bf16)x;
+            d[8 * k] = part;
+            x -= (float)part;
+        }
+        float *gr = (float *)(basis_tile + (size_t)tp * lda * (2 * kAParts)) + row * (size_t)lda + f;
         gr[0] = (float)gx;
         gr[(size_t)tp * lda] = (float)gy;
         gr[2 * (size_t)tp * lda] = (float)gz;
@@ -287,12 +288,12 @@ __global__ __launch_bounds__(256) void basis_eval_kernel(EvalArgs a) {
 template <int LMAX, bool F64 = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LMAX <= 3 ? 4 : 1)))
 void basis_eval_staged_kernel(EvalArgs a) {
-    constexpr int kFunBytes = F64 ? 32 : 18;  // image bytes per function
+    constexpr int kFunBytes = F64 ? 32 : 2 * kAParts + 12;  // image bytes per function
     const int tid = threadIdx.x, tp = a.tile_points;
     const size_t row0 = (size_t)blockIdx.x * tp;
     int wt = a.lda;  // functions per row of this tile's output
     short *bpos = nullptr;
-    __bf16 *phi_dst = a.phis + row0 * 3 * (size_t)a.lda;
+    __bf16 *phi_dst = a.phis + row0 * kAParts * (size_t)a.lda;
     float *grad_dst = a.grad32 + row0 * (size_t)a.lda;
     size_t grad_plane = (size_t)a.gplane;
     if (!F64 && a.toff) {
@@ -306,7 +307,7 @@ void basis_eval_staged_kernel(EvalArgs a) {
         for (int i = tid; i < na; i += 256) bpos[a.blist[(size_t)t * a.nblk + i]] = (short)i;
         const size_t base = (size_t)a.toff[t] * a.tile_rows * 32;
         const size_t rin = row0 - (size_t)t * a.tile_rows;
-        phi_dst = a.phis + base * 3 + rin * 3 * (size_t)wt;
+        phi_dst = a.phis + base * kAParts + rin * kAParts * (size_t)wt;
         grad_dst = a.grad32 + base * 3 + rin * (size_t)wt;
         grad_plane = (size_t)a.tile_rows * wt;
     }
@@ -318,7 +319,7 @@ void basis_eval_staged_kernel(EvalArgs a) {
         EvalArgs b = a;
         b.lda = wt;         // row stride of the LDS image
         b.tile_points = nr; // rows in the image
-        const size_t phi_bytes = (size_t)nr * wt * (F64 ? 8 : 6),
+        const size_t phi_bytes = (size_t)nr * wt * (F64 ? 8 : 2 * kAParts),
                      grad_bytes = (size_t)nr * wt * (F64 ? 24 : 12);
         {
             uint4 *z = (uint4 *)basis_tile;
@@ -370,7 +371,7 @@ void basis_eval_staged_kernel(EvalArgs a) {
         }
         {
             const uint4 *src = (const uint4 *)basis_tile;
-            uint4 *dst = (uint4 *)(phi_dst + (size_t)r0 * 3 * wt);
+            uint4 *dst = (uint4 *)(phi_dst + (size_t)r0 * kAParts * wt);
             for (size_t i = tid; i < phi_bytes / 16; i += 256) dst[i] = src[i];
         }
         for (int d = 0; d < 3; d++) {
@@ -424,7 +425,7 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
         // staged in passes through an image of 36 KiB or one dense row,
         // whichever is larger (four workgroups per CU at nbf <= 2048)
         EvalArgs b = a;
-        const size_t row_bytes = (size_t)a.lda * 18;
+        const size_t row_bytes = (size_t)a.lda * (2 * kAParts + 12);
         int tp = 8;
         size_t image;
         if (a.toff) {

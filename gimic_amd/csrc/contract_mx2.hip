@@ -1,22 +1,23 @@
 // Mixed-precision contraction with pre-split operands (gfx950).
 //
-// Arithmetic as in contract_bf16x6_kernel: each FP32 operand value is the
-// exact sum of three BF16 parts h + m + l, a block product keeps the six
-// terms hh, hm, mh, hl, mm, lh (smallest first) on
-// v_mfma_f32_16x16x32_bf16 with FP32 accumulation, and FP64 partial sums
-// are kept across N-tiles.  What differs is where the splitting happens:
-//   * the basis kernel writes the left operand already split, [row][k/8][3][8]
-//     (48 contiguous bytes per lane and K-step), and
+// Arithmetic: an FP32 operand value is the exact sum of three BF16 parts
+// h + m + l; the basis values keep kAParts of them and the right operand
+// kBParts (device_common.h), and a block product keeps the part products of
+// split order <= 2 (three and three: hh, hm, mh, hl, mm, lh), smallest
+// first, on v_mfma_f32_16x16x32_bf16 with FP32 accumulation; FP64 partial
+// sums are kept across N-tiles.  Both operands arrive split:
+//   * the basis kernel writes the left operand as [row][k/8][kAParts][8]
+//     (16 kAParts contiguous bytes per lane and K-step), and
 //   * the right operand (constant per density) is split once on the host into
-//     10.5 KiB groups per (density set, N-tile, 8-function group),
-//     [part][column][8]; a K-step's 44 KiB LDS buffer holds four of them.
+//     groups of kBParts x 3.5 KiB per (density set, N-tile, 8-function
+//     group), [part][column][8]; a K-step's LDS buffer holds four of them.
 // The K-loop therefore has no conversion work: each K-step streams the four
 // groups of a later step into a free LDS buffer with gfx950's direct
-// global-to-LDS loads (global_load_lds_dwordx4: no VGPRs, no ds_write), loads the next A
-// fragments, and runs 168 MFMAs per wavefront against B fragments that are
-// read from LDS one column block ahead.  NBUF = 2 streams block ks+1 during
-// K-step ks; NBUF = 3 streams block ks+2 (154 KiB of LDS), so a block has
-// two K-steps to arrive from the Infinity Cache.
+// global-to-LDS loads (global_load_lds_dwordx4: no VGPRs, no ds_write), loads
+// the next A fragments, and runs KRB x 14 x split_products() MFMAs per
+// wavefront against B fragments that are read from LDS one column block
+// ahead.  NBUF = 2 streams block ks+1 during K-step ks; NBUF = 3 streams
+// block ks+2, so a block has two K-steps to arrive from the Infinity Cache.
 //
 // Tile: 16 * KRB * WAVES point rows per workgroup (WAVES wavefronts x 16 KRB
 // rows), 224 columns per N-tile (7 matrices x 32 functions), KRB x 14 MFMA
@@ -33,7 +34,7 @@
 // no-return atomic adds, 16 lanes of a group covering 128 contiguous bytes.
 // Each address belongs to one lane of one workgroup, which adds its N-tiles
 // in order to a buffer cleared before the launch, so the sums are the same,
-// bit for bit, on every run, and no LDS holds them (three 44 KiB buffers fit
+// bit for bit, on every run, and no LDS holds them (three K-step buffers fit
 // with 256-row tiles).
 
 //
@@ -71,8 +72,18 @@ constexpr int kNu = 32;
 constexpr int kCols = kMx2Cols;             // 224
 constexpr int kCB = kCols / 16;             // 14 column blocks
 constexpr int kBlk = kMx2BlockElems;        // BF16 per block / LDS buffer
-constexpr int kPieces = kBlk * 2 / 1024;    // 1 KiB pieces per block (44)
 constexpr size_t kBufBytes = (size_t)kBlk * 2;
+// A K-step's LDS buffer is [group slot j][part][column][8]: slot j holds one
+// 8-function group (kBParts x 3584 bytes, contiguous in HBM too), copied in
+// kPPG 1 KiB pieces per slot, the last one partial when the group does not
+// fill it (three parts: 10752 bytes, eleven pieces, the last for lanes 0-31)
+constexpr int kGroupBytes = kMx2GroupElems * 2;
+constexpr int kPPG = (kGroupBytes + 1023) / 1024;           // pieces per group slot
+constexpr int kLastLanes = (kGroupBytes - (kPPG - 1) * 1024) / 16;  // lanes of its last piece
+constexpr int kPieces = 4 * kPPG;                            // pieces per K-step
+constexpr int kPartElems = kCols * 8;                       // BF16 per part of a group
+static_assert(4 * kGroupBytes <= kBlk * 2 && kGroupBytes % 16 == 0, "piece layout");
+
 // pieces a wavefront copies of each block: wavefront w copies w, w + WAVES, ...
 template <int WAVES>
 constexpr int pieces_max() { return (kPieces + WAVES - 1) / WAVES; }
@@ -82,27 +93,19 @@ __device__ __forceinline__ int pieces_of(int wave, int waves) {
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// A K-step's LDS buffer is [group slot j][part][column][8]: slot j holds one
-// 8-function group (10752 bytes, contiguous in HBM too), so its 44 pieces are
-// 11 per slot, the eleventh a half piece (lanes 0-31)
-constexpr int kGroupBytes = kMx2GroupElems * 2;   // 10752
-constexpr int kPartElems = kCols * 8;             // BF16 per part of a group
-static_assert(4 * kGroupBytes <= kBlk * 2 && kPieces == 44 && kGroupBytes == 10 * 1024 + 512,
-              "piece layout");
-
 // piece i of this wavefront's share of a K-step's buffer (64 lanes x 16
-// bytes per instruction, LDS base in M0): piece q = 11 j + t copies bytes
+// bytes per instruction, LDS base in M0): piece q = kPPG j + t copies bytes
 // [1024 t, 1024 t + 1024) of group slot j from global group ent[j] >> 12 of
 // the N-tile (ent[j] < 0: from the zero buffer)
 template <int WAVES>
 __device__ __forceinline__ void stage_piece_k8(const __bf16 *Bnt, int4 ent, const __bf16 *bzero,
                                                unsigned char *buf, int wave, int lane, int i) {
     const int piece = wave + WAVES * i;
-    const int j = piece / 11, t = piece - 11 * j;
+    const int j = piece / kPPG, t = piece - kPPG * j;
     const int v = j == 0 ? ent.x : j == 1 ? ent.y : j == 2 ? ent.z : ent.w;
     const __bf16 *grp = v < 0 ? bzero : Bnt + (size_t)(v >> 12) * kMx2GroupElems;
     const unsigned char *src = (const unsigned char *)grp + t * 1024 + lane * 16;
-    if (t < 10 || lane < 32)
+    if (t < kPPG - 1 || lane < kLastLanes)
         __builtin_amdgcn_global_load_lds((const void *)src,
                                          (lds_void *)(buf + j * kGroupBytes + t * 1024), 16, 0, 0);
 }
@@ -140,13 +143,18 @@ __device__ __forceinline__ void row16_xsum(float (&v)[32], int c) {
     xsum_step<2, 0xB1>(v, (c & 1) != 0);
 }
 
-__device__ __forceinline__ v4f mfma6(const v8bf (&a)[3], const v8bf (&b)[3], v4f acc) {
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
+// the part products of split order <= 2, smallest first (three parts each:
+// lh, mm, hl, mh, hm, hh)
+__device__ __forceinline__ v4f mfma_split(const v8bf (&a)[kAParts], const v8bf (&b)[kBParts],
+                                          v4f acc) {
+#pragma unroll
+    for (int ord = 2; ord >= 0; ord--)
+#pragma unroll
+        for (int ia = kAParts - 1; ia >= 0; ia--) {
+            const int ib = ord - ia;
+            if (ib >= 0 && ib < kBParts)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ia], b[ib], acc, 0, 0, 0);
+        }
     return acc;
 }
 
@@ -202,8 +210,9 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
     // the gradients are addressed by list position)
     const bool cmp = a.toff != nullptr;
     const size_t wcols = cmp ? (size_t)kNu * na : (size_t)lda;  // functions per row
-    const size_t arow = 3 * wcols;                              // BF16 per split row
-    const __bf16 *Atile = cmp ? a.as3 + (size_t)a.toff[rtile] * kRows * 96 : a.as3 + row0 * arow;
+    const size_t arow = kAParts * wcols;                        // BF16 per split row
+    const __bf16 *Atile = cmp ? a.as3 + (size_t)a.toff[rtile] * kRows * (kNu * kAParts)
+                              : a.as3 + row0 * arow;
     const float *Gtile = cmp ? a.g32 + (size_t)a.toff[rtile] * kRows * 96 : a.g32 + row0 * wcols;
     const size_t gpl = cmp ? (size_t)kRows * wcols : (size_t)a.gplane;  // gradient plane stride
     const int np = pieces_of(wave, WAVES);  // pieces this wavefront copies per block
@@ -222,7 +231,7 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
     const __bf16 *Arow = Atile + (size_t)(wave * 16 * kRB + c) * arow;
     auto a_off = [&](int4 v) -> size_t {  // this lane's group of a K-step, in BF16 of a row
         const int x = g == 0 ? v.x : g == 1 ? v.y : g == 2 ? v.z : v.w;
-        return x < 0 ? 0 : (size_t)(cmp ? (x & 4095) : (x >> 12)) * 24;
+        return x < 0 ? 0 : (size_t)(cmp ? (x & 4095) : (x >> 12)) * (8 * kAParts);
     };
     const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kMx2StepElems;
     const int e0 = (int)((long long)split * na / a.nsplit);
@@ -240,13 +249,14 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
         const int4 q0 = *(const int4 *)K8, q1 = *(const int4 *)(K8 + (nq > 1 ? 4 : 0));
         stage_block_k8<WAVES>(Bnt, q0, a.bzero, bbuf, wave, lane);
         if (NBUF == 3 && nq > 1) stage_block_k8<WAVES>(Bnt, q1, a.bzero, bbuf + kBufBytes, wave, lane);
-        v8bf an[kRB][3];
+        v8bf an[kRB][kAParts];
         {
             const size_t ao = a_off(q0);
 #pragma unroll
             for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
-                for (int p = 0; p < 3; p++) an[rb][p] = *(const v8bf *)(Arow + rb * 16 * arow + ao + p * 8);
+                for (int p = 0; p < kAParts; p++)
+                    an[rb][p] = *(const v8bf *)(Arow + rb * 16 * arow + ao + p * 8);
         }
         __builtin_amdgcn_s_waitcnt(kWaitAll);
         __syncthreads();
@@ -259,11 +269,11 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
             const int4 ea = ent_a, ec = ent_c;
             ent_a = *(const int4 *)(K8 + 4 * min(ks + 2, nq - 1));
             ent_c = *(const int4 *)(K8 + 4 * min(ks + NBUF, nq - 1));
-            v8bf as[kRB][3];
+            v8bf as[kRB][kAParts];
 #pragma unroll
             for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
-                for (int p = 0; p < 3; p++) as[rb][p] = an[rb][p];
+                for (int p = 0; p < kAParts; p++) as[rb][p] = an[rb][p];
             // Block streamed during this K-step.  With NBUF = 3 the copy is
             // issued every K-step (the last two repeat the last listed block
             // into a buffer nobody reads again) so that the count of outstanding
@@ -277,7 +287,7 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
             // loads (column blocks 3 to 13), so that waiting for "all but the
             // last copy" covers the A loads.
             const bool more = ks + 1 < nq;
-            constexpr int kALoads = 3 * kRB;
+            constexpr int kALoads = kAParts * kRB;
             constexpr int kCopyFrom = NBUF == 3 ? kCB - kPW : 0;
             static_assert(NBUF != 3 || (kALoads + 1) / 2 <= kCopyFrom, "A loads must precede the copy");
             const int ahead = ks + NBUF - 1;
@@ -285,11 +295,12 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
             const size_t aoff = a_off(ea);
             unsigned char *cbuf = bbuf + (size_t)(ahead % NBUF) * kBufBytes;
             const __bf16 *cur = (const __bf16 *)(bbuf + (size_t)(ks % NBUF) * kBufBytes);
-            v8bf bf[2][3];
+            v8bf bf[2][kBParts];
             {
                 const int off = g * kMx2GroupElems + c * 8;
 #pragma unroll
-                for (int p = 0; p < 3; p++) bf[0][p] = *(const v8bf *)(cur + p * kPartElems + off);
+                for (int p = 0; p < kBParts; p++)
+                    bf[0][p] = *(const v8bf *)(cur + p * kPartElems + off);
             }
 #pragma unroll
             for (int cb = 0; cb < kCB; cb++) {
@@ -297,19 +308,19 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
                 if (cb + 1 < kCB) {
                     const int off = g * kMx2GroupElems + ((cb + 1) * 16 + c) * 8;
 #pragma unroll
-                    for (int p = 0; p < 3; p++)
+                    for (int p = 0; p < kBParts; p++)
                         bf[sl ^ 1][p] = *(const v8bf *)(cur + p * kPartElems + off);
                 }
 #pragma unroll
                 for (int j = 2 * cb; j < 2 * cb + 2; j++)
                     if (more && j < kALoads)
-                        an[j / 3][j % 3] =
-                            *(const v8bf *)(Arow + (j / 3) * 16 * arow + aoff + (j % 3) * 8);
+                        an[j / kAParts][j % kAParts] = *(const v8bf *)(
+                            Arow + (j / kAParts) * 16 * arow + aoff + (j % kAParts) * 8);
                 if (copy && cb >= kCopyFrom && cb - kCopyFrom < np)
                     stage_piece_k8<WAVES>(Bnt, ec, a.bzero, cbuf, wave, lane, cb - kCopyFrom);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int rb = 0; rb < kRB; rb++) acc[rb][cb] = mfma6(as[rb], bf[sl], acc[rb][cb]);
+                for (int rb = 0; rb < kRB; rb++) acc[rb][cb] = mfma_split(as[rb], bf[sl], acc[rb][cb]);
                 __builtin_amdgcn_sched_barrier(0);
             }
             // this wavefront's A loads and the copy of block ks+1 have landed
@@ -346,17 +357,17 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
             for (int d = 0; d < 3; d++) R[h][d] = a.rf32[(size_t)d * a.ncol_pad + nt * kNu + 16 * h + c];
         constexpr int kIt = 4 * kRB;  // rows per lane: it = 4 rb + r
         auto row_of = [&](int it) { return wave * 16 * kRB + (it >> 2) * 16 + 4 * g + (it & 3); };
-        __bf16 pc[2][3], pn[2][3];
+        __bf16 pc[2][kAParts], pn[2][kAParts];
         float Gc[2][3], Gn[2][3];
-        auto load_row = [&](int it, __bf16 (&pv)[2][3], float (&Gv)[2][3]) {
+        auto load_row = [&](int it, __bf16 (&pv)[2][kAParts], float (&Gv)[2][3]) {
             const size_t rl = row_of(it);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 // column of function nt*32 + 16h + c in this tile's basis rows
                 const int acol = (cmp ? e : nt) * kNu + 16 * h + c;
-                const __bf16 *pe = Atile + rl * arow + (acol >> 3) * 24 + (acol & 7);
+                const __bf16 *pe = Atile + rl * arow + (acol >> 3) * (8 * kAParts) + (acol & 7);
 #pragma unroll
-                for (int k = 0; k < 3; k++) pv[h][k] = pe[8 * k];
+                for (int k = 0; k < kAParts; k++) pv[h][k] = pe[8 * k];
                 const size_t gi = rl * wcols + acol;
 #pragma unroll
                 for (int d = 0; d < 3; d++) Gv[h][d] = Gtile[d * gpl + gi];
@@ -375,7 +386,10 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
                 for (int i = 0; i < kNQ; i++) q[i] = 0.f;
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                    const float ph = ((float)pc[h][0] + (float)pc[h][1]) + (float)pc[h][2];
+                    // the basis value from its parts (three: exactly the FP32 value)
+                    float ph = (float)pc[h][0];
+#pragma unroll
+                    for (int k = 1; k < kAParts; k++) ph += (float)pc[h][k];
                     const float *G = Gc[h];
                     const float t0 = acc[rb][h][r] * ph;
                     q[0] += t0;
@@ -405,12 +419,12 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
                 if (c + 16 < kNQ) unsafeAtomicAdd(o + c + 16, (double)s1);
             }
 #pragma unroll
-            for (int h = 0; h < 2; h++)
+            for (int h = 0; h < 2; h++) {
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    pc[h][k] = pn[h][k];
-                    Gc[h][k] = Gn[h][k];
-                }
+                for (int k = 0; k < kAParts; k++) pc[h][k] = pn[h][k];
+#pragma unroll
+                for (int k = 0; k < 3; k++) Gc[h][k] = Gn[h][k];
+            }
             __builtin_amdgcn_sched_barrier(0);
         }
     }

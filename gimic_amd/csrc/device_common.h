@@ -90,9 +90,9 @@ struct EvalArgs {
     int lda;             // padded K (multiple of kKPad)
     // FP64 outputs
     double *phi64, *grad64;  // [rows][lda], [3][rows][lda]
-    // mixed outputs: FP32 gradients, and each value x as three BF16 parts
-    // x = h + m + l (exact), layout [row][lda/8][3][8] (compressed to the
-    // listed blocks of each row tile, below)
+    // mixed outputs: FP32 gradients, and each value x as kAParts BF16 parts
+    // (x = h + m + l exactly with three), layout [row][lda/8][kAParts][8]
+    // (compressed to the listed blocks of each row tile, below)
     float *grad32;
     __bf16 *phis;
     int tile_points;     // points per workgroup of the LDS-staged kernel (set by the launcher)
@@ -100,8 +100,9 @@ struct EvalArgs {
     int lmax;            // largest shell l of the basis
     // Compressed split output (staged kernel, toff non-null): a row tile of
     // tile_rows rows keeps only its bcount[t] listed 32-function blocks,
-    // blist[t * nblk ...]; its rows start at element toff[t] * tile_rows * 96
-    // of phis ([row][32 n / 8][3][8]) and of grad32 ([3][tile_rows][32 n])
+    // blist[t * nblk ...]; its rows start at element toff[t] * tile_rows *
+    // 32 kAParts of phis ([row][32 n / 8][kAParts][8]) and toff[t] *
+    // tile_rows * 96 of grad32 ([3][tile_rows][32 n])
     const int *bcount, *blist, *toff;
     int nblk, tile_rows;
 };
@@ -117,7 +118,7 @@ struct ContractArgs {
     int npts;
     const double *a64, *g64, *b64;  // FP64 path
     const float *g32;    // mixed path: FP32 gradients
-    const __bf16 *as3;   // split basis values [rows][lda/8][3][8] (mixed kernels)
+    const __bf16 *as3;   // split basis values [rows][lda/8][kAParts][8] (mixed kernels)
     const __bf16 *bs3;   // split right operand, LDS-image blocks per (set, N-tile, K-step)
     const double *rf64;  // [3][ncol_pad] atom coordinate of each function
     const float *rf32;
@@ -142,14 +143,27 @@ hipError_t launch_contract_f64(const ContractArgs &a, hipStream_t s);
 // 2 or 3, or krb 4, nbuf 2; 8 waves with krb 2, nbuf 2 or 3).  Adds its
 // partial sums to a.part, row-major, which must be zero before the launch.
 hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, int waves, hipStream_t s);
-constexpr int kMx2Cols = 224;                         // 7 matrices x 32 functions
-constexpr int kMx2BlockElems = 22528;                 // LDS buffer: 3*4*224*8 = 21504, padded to 44 KiB
-// pre-split right operand in HBM: per (density set, N-tile, 8-function group
-// kg) one group of 3 parts x 224 columns x 8 (10752 bytes), [part][column][8];
-// the four groups of a 32-function K-block are consecutive (a K-step image,
-// [group][part][column][8], 43008 bytes), plus 2 KiB of slack at the end
-constexpr int kMx2GroupElems = 3 * kMx2Cols * 8;      // 5376
-constexpr int kMx2StepElems = 4 * kMx2GroupElems;     // 21504
+// Mixed arithmetic: an FP32 operand value x is the sum of BF16 parts
+// x = h + m + l (three parts: exact); the basis values (left operand) keep
+// kAParts of them and the right operand kBParts, and a block product keeps
+// the part products of split order <= 2 (h = 0, m = 1, l = 2)
+constexpr int kAParts = 3;
+constexpr int kBParts = 3;
+constexpr int split_products() {
+    int n = 0;
+    for (int i = 0; i < kAParts; i++)
+        for (int j = 0; j < kBParts; j++) n += i + j <= 2;
+    return n;
+}
+constexpr int kMx2Cols = 224;                             // 7 matrices x 32 functions
+// right operand in HBM: per (density set, N-tile, 8-function group kg) one
+// group of kBParts parts x 224 columns x 8, [part][column][8]; the four
+// groups of a 32-function K-block are consecutive (a K-step image,
+// [group][part][column][8]), plus 2 KiB of slack at the end
+constexpr int kMx2GroupElems = kBParts * kMx2Cols * 8;   // 5376 for three parts
+constexpr int kMx2StepElems = 4 * kMx2GroupElems;        // 21504
+// the kernel's LDS buffer of one K-step: four groups in whole KiB
+constexpr int kMx2BlockElems = (4 * kMx2GroupElems * 2 + 1023) / 1024 * 512;
 
 struct FinalizeArgs {
     const double *part;  // [nmatset][nsplit][kNQ][rows], or [nmatset][nsplit][rows][kNQ] if part_rows

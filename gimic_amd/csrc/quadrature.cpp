@@ -1,11 +1,18 @@
-// Gauss-Legendre / Gauss-Lobatto nodes by Newton iteration on Legendre
-// polynomials from the three-term recurrence, as gaussint.f90 does (same
-// initial guesses, tolerance 3e-12 and at most 10 Newton steps), so the
-// nodes and weights agree with the reference to rounding.
+// Host-side Gauss-Legendre and Gauss-Lobatto rules (gaussint.f90:21-112,
+// composite rule gaussint.f90:267-319).
+//
+// The nodes are roots of P_n (Legendre) or of P'_{n-1} (Lobatto interior
+// nodes), found by Newton's method from the Chebyshev-like start
+// cos(pi (i - 1/4) / (n + 1/2)).  The flux fixtures compare quadrature sums
+// to the reference's sixth decimal and the device plane generator equals
+// the host grids bit for bit, so the start values, the step tolerance, the
+// step limit and the order of every product and sum follow the reference:
+// then the nodes agree with it to the last bit, not just to rounding.
 #include "quadrature.h"
 
 #include <cmath>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace gimic {
@@ -13,80 +20,97 @@ namespace gimic {
 namespace {
 
 constexpr double kPi = 3.141592653589793;  // globals.f90 PII
-constexpr double kEps = 3.0e-12;           // gaussint.f90:16
-constexpr int kNewtonMax = 10;             // gaussint.f90:17
+constexpr double kRootTol = 3.0e-12;       // Newton step that ends the search
+constexpr int kMaxSteps = 10;              // a root needing the 10th step is a failure
 
-// P_n(x), P_n'(x), P_n''(x) by the recurrence i P_i = (2i-1) x P_{i-1} - (i-1) P_{i-2}
-void legendre(double x, int n, double &y, double &dy, double &d2y) {
-    y = 1.0;
-    dy = 0.0;
-    d2y = 0.0;
-    if (n == 0) return;
-    y = x;
-    dy = 1.0;
-    if (n == 1) return;
-    double yp = 1.0, dyp = 0.0, d2yp = 0.0;
-    for (int i = 2; i <= n; i++) {
-        const double c1 = (double)i, c2 = c1 * 2.0 - 1.0, c4 = c1 - 1.0;
-        const double ym = y;
-        y = (c2 * x * y - c4 * yp) / c1;
-        yp = ym;
-        const double dym = dy;
-        dy = (c2 * x * dy - c4 * dyp + c2 * yp) / c1;
-        dyp = dym;
-        const double d2ym = d2y;
-        d2y = (c2 * x * d2y - c4 * d2yp + c2 * 2.0 * dyp) / c1;
-        d2yp = d2ym;
+struct LegendreValue {
+    double p, dp, ddp;  // P_n(x), P_n'(x), P_n''(x)
+};
+
+// Bonnet's recurrence k P_k = (2k - 1) x P_{k-1} - (k - 1) P_{k-2} and its
+// first two derivatives, differentiated term by term; each line is
+// evaluated left to right as written.
+LegendreValue legendre_at(int degree, double x) {
+    LegendreValue cur{1.0, 0.0, 0.0};
+    if (degree == 0) return cur;
+    LegendreValue prev = cur;
+    cur = {x, 1.0, 0.0};
+    for (int k = 2; k <= degree; k++) {
+        const double kk = (double)k, two_k_1 = kk * 2.0 - 1.0, k_1 = kk - 1.0;
+        LegendreValue next;
+        next.p = (two_k_1 * x * cur.p - k_1 * prev.p) / kk;
+        next.dp = (two_k_1 * x * cur.dp - k_1 * prev.dp + two_k_1 * cur.p) / kk;
+        next.ddp = (two_k_1 * x * cur.ddp - k_1 * prev.ddp + two_k_1 * 2.0 * cur.dp) / kk;
+        prev = cur;
+        cur = next;
     }
+    return cur;
+}
+
+double start_value(int i, int n) { return std::cos(kPi * (i - 0.25) / (n + 0.5)); }
+
+// Newton from x0: `step(x, &v)` returns the new iterate and the Legendre
+// values it used; the search ends when an iterate moves by at most kRootTol.
+template <class Step>
+double newton_root(double x0, Step step, LegendreValue &at_root, const char *rule) {
+    double x = x0;
+    int n = 1;
+    for (; n <= kMaxSteps; n++) {
+        const double before = x;
+        x = step(before, at_root);
+        if (std::fabs(x - before) <= kRootTol) break;
+    }
+    if (n >= kMaxSteps)
+        throw std::runtime_error(std::string(rule) + ": no node within " +
+                                 std::to_string(kMaxSteps) + " Newton steps");
+    return x;
 }
 
 }  // namespace
 
 void gauss_legendre(double a, double b, int n, double *pts, double *wgts) {
-    const int m = (n + 1) / 2;
-    const double xm = (b + a) * 0.5, xl = (b - a) * 0.5;
-    for (int i = 1; i <= m; i++) {
-        double z = std::cos(kPi * (i - 0.25) / (n + 0.5));
-        double lp = 0.0, dlp = 0.0, d2 = 0.0;
-        int it = 1;
-        for (; it <= kNewtonMax; it++) {
-            legendre(z, n, lp, dlp, d2);
-            const double z1 = z;
-            z = z1 - lp / dlp;
-            if (std::fabs(z - z1) <= kEps) break;
-        }
-        if (it >= kNewtonMax) throw std::runtime_error("integration did not converge");
-        pts[i - 1] = xm - xl * z;
-        pts[n - i] = xm + xl * z;
-        wgts[i - 1] = 2.0 * xl / ((1.0 - z * z) * dlp * dlp);
+    const double mid = (b + a) * 0.5, half = (b - a) * 0.5;
+    for (int i = 1; i <= (n + 1) / 2; i++) {
+        LegendreValue v{};
+        const double z = newton_root(
+            start_value(i, n),
+            [n](double x, LegendreValue &lv) {
+                lv = legendre_at(n, x);
+                return x - lv.p / lv.dp;
+            },
+            v, "Gauss-Legendre rule");
+        pts[i - 1] = mid - half * z;
+        pts[n - i] = mid + half * z;
+        wgts[i - 1] = 2.0 * half / ((1.0 - z * z) * (v.dp * v.dp));
         wgts[n - i] = wgts[i - 1];
     }
 }
 
 void gauss_lobatto(double a, double b, int n, double *pts, double *wgts) {
-    const double xm = (b + a) * 0.5, xl = (b - a) * 0.5;
-    pts[0] = xm - xl;
-    pts[n - 1] = xm + xl;
-    wgts[0] = 2.0 * xl / (double)(n * n - n);
+    const double mid = (b + a) * 0.5, half = (b - a) * 0.5;
+    const double nn = (double)(n * n - n);
+    pts[0] = mid - half;
+    pts[n - 1] = mid + half;
+    wgts[0] = 2.0 * half / nn;
     wgts[n - 1] = wgts[0];
     for (int i = 2; i <= n - 1; i++) {
-        double z = std::cos(kPi * (i - 0.25) / (n + 0.5));
-        double lp = 0.0, dlp = 0.0, d2lp = 0.0;
-        int it = 1;
-        for (; it <= kNewtonMax; it++) {
-            legendre(z, n - 1, lp, dlp, d2lp);
-            const double z1 = z;
-            z = z1 - dlp / d2lp;
-            double damp = 0.5;
-            while (std::fabs(z) > 1.0) {  // damped step back into [-1,1]
-                z = z1 - dlp / d2lp * damp;
-                damp = damp * damp;
-            }
-            if (std::fabs(z - z1) <= kEps) break;
-        }
-        if (it >= kNewtonMax) throw std::runtime_error("lobatto Newton step did not converge");
-        pts[i - 1] = xm - xl * z;
-        wgts[i - 1] = 2.0 * xl / ((double)(n * n - n) * lp * lp);
+        LegendreValue v{};
+        const double z = newton_root(
+            start_value(i, n),
+            [n](double x, LegendreValue &lv) {
+                lv = legendre_at(n - 1, x);
+                // a step that leaves [-1, 1] is shortened by factors 1/2,
+                // 1/4, 1/16, ... of itself until it does not
+                double next = x - lv.dp / lv.ddp, shrink = 0.5;
+                while (std::fabs(next) > 1.0) {
+                    next = x - lv.dp / lv.ddp * shrink;
+                    shrink = shrink * shrink;
+                }
+                return next;
+            },
+            v, "Gauss-Lobatto rule");
+        pts[i - 1] = mid - half * z;
+        wgts[i - 1] = 2.0 * half / (nn * (v.p * v.p));
     }
 }
 
@@ -97,18 +121,21 @@ void setup_gauss_data(double a, double b, int order, int npts, bool lobatto, dou
         wgts[0] = 1.0;
         return;
     }
-    if (npts % order != 0) throw std::runtime_error("npts is not dividable by ngp");
-    const int nblock = npts / order;
-    const double step = (b - a) / (double)nblock;
-    const double xl = step * 0.5;
-    std::vector<double> tp(order), tw(order);
-    if (lobatto) gauss_lobatto(-1.0, 1.0, order, tp.data(), tw.data());
-    else gauss_legendre(-1.0, 1.0, order, tp.data(), tw.data());
-    int k = 0;
-    for (int i = 0; i < nblock; i++)
+    if (npts % order != 0)
+        throw std::runtime_error("quadrature: " + std::to_string(npts) +
+                                 " points are not a whole number of blocks of " +
+                                 std::to_string(order));
+    const int blocks = npts / order;
+    const double width = (b - a) / (double)blocks, half = width * 0.5;
+    std::vector<double> node(order), weight(order);
+    if (lobatto) gauss_lobatto(-1.0, 1.0, order, node.data(), weight.data());
+    else gauss_legendre(-1.0, 1.0, order, node.data(), weight.data());
+    // block i covers [i width, (i + 1) width]; Lobatto blocks repeat their
+    // shared end points
+    for (int i = 0, k = 0; i < blocks; i++)
         for (int j = 0; j < order; j++, k++) {
-            pts[k] = (tp[j] + 1.0) * xl + (double)i * step;
-            wgts[k] = tw[j] * xl;
+            pts[k] = (node[j] + 1.0) * half + (double)i * width;
+            wgts[k] = weight[j] * half;
         }
 }
 

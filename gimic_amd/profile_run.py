@@ -21,8 +21,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--kernel", type=int, default=0,
-                    help="mixed contraction kernel 1..3 (0: automatic selection)")
-    ap.add_argument("--split", type=int, default=0, help="N-split count of kernel 3")
+                    help="mixed contraction kernel 3, 4, 5, 8 or 9 (0: automatic selection)")
+    ap.add_argument("--split", type=int, default=0, help="N-split count of the mixed kernel")
     a = ap.parse_args()
     import torch
     from . import synth

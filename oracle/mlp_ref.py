@@ -96,8 +96,11 @@ def predict(params, freqs, center_scale, out_scale, r, bf=False):
     return y[:, :NOUT] * amp(y)[:, None] * np.asarray(out_scale, dtype=np.float64)
 
 
-def loss_grad(params, freqs, center_scale, out_scale, r, target, eps, bf=False):
-    """(loss, flat gradient) for one batch"""
+def loss_grad(params, freqs, center_scale, out_scale, r, target, eps, bf=False, plant=None):
+    """(loss, flat gradient) for one batch.  plant = (kind, layer) puts a
+    known mistake into one layer's backward pass, for tests that show a
+    bound catches it: "no_dsilu" drops SiLU' where the gradient leaves the
+    layer's input, "no_bias" leaves its bias gradient at zero"""
     layers = unpack(params)
     rnd = bf16 if bf else (lambda v: v)
     y, (xs, zs) = forward(params, freqs, center_scale, r, bf)
@@ -117,8 +120,8 @@ def loss_grad(params, freqs, center_scale, out_scale, r, target, eps, bf=False):
         woff, boff, kin, nout = lay[l]
         W, _ = layers[l]
         grad[woff:boff] = (rnd(g).T @ xs[l]).ravel()
-        grad[boff:boff + nout] = g.sum(axis=0)
+        grad[boff:boff + nout] = 0.0 if plant == ("no_bias", l) else g.sum(axis=0)
         if l > 0:
             dh = rnd(g) @ rnd(W)
-            g = dh * dsilu(zs[l - 1])
+            g = dh if plant == ("no_dsilu", l) else dh * dsilu(zs[l - 1])
     return loss, grad
