@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1082,6 +1083,47 @@ void refine_near_nuclei(gimic_ctx *c, int64_t npts, const double *d_r, double *d
     c->refined_last = k;
 }
 
+// ---- allocation-free evaluation (gimic_jtensor_batch_device_noalloc) ----
+
+// Work memory of the allocation-free entry point: one copy on every GPU the
+// library runs on, held until the process ends.  Host code uses it only
+// through the device address hipGetSymbolAddress returns (scratch_address).
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current
+unsigned char *scratch_address() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw Error(GIMIC_EDEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw Error(GIMIC_EDEVICE, "no device address for the scratch array");
+        g_scratch_addr[dev] = p;
+    }
+    return (unsigned char *)g_scratch_addr[dev];
+}
+
+// Makes `dev` current for the guard's lifetime
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        hip_check(hipGetDevice(&prev), "hipGetDevice");
+        if (prev != dev) hip_check(hipSetDevice(dev), "hipSetDevice");
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
 // ---- drop-in global state (gimic_interface.f90:21-24) ----
 gimic_ctx *g_ctx = nullptr;
 double g_magnet[3] = {0.0, 0.0, 0.0};
@@ -1249,6 +1291,57 @@ int gimic_jtensor_multi_device(gimic_ctx *c, int64_t npts, const double *d_r, do
 int gimic_jtensor_batch_device(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
                                int spin, int precision, void *stream) {
     return gimic_jtensor_multi_device(c, npts, d_r, d_jt, 1, &spin, precision, stream);
+}
+
+int gimic_jtensor_batch_device_noalloc(gimic_ctx *c, int64_t npts, const double *d_r,
+                                       double *d_jt, int spin, int precision, void *stream) {
+    return guarded([&] {
+        if (!c || npts < 0 || (npts > 0 && (!d_r || !d_jt))) throw Error(GIMIC_EINVAL, "bad argument");
+        if (spin < 0 || spin > 3) throw Error(GIMIC_EINVAL, "invalid spin case");
+        if (precision != GIMIC_PREC_FP64 && precision != GIMIC_PREC_MIXED)
+            throw Error(GIMIC_EINVAL, "unknown precision");
+        if (npts == 0) return;
+        DeviceGuard device(c->device);
+        hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+        const std::vector<int> spins{spin};
+        const bool refine = precision == GIMIC_PREC_MIXED && c->fp64_radius > 0.0;
+        if (!c->mats.count(std::make_pair(spins, precision)) ||
+            (refine && !c->mats.count(std::make_pair(spins, (int)GIMIC_PREC_FP64))))
+            throw Error(GIMIC_ESTATE,
+                        "right operands of this spin case and precision are not resident: one "
+                        "gimic_jtensor_batch_device call with them (one point suffices) builds "
+                        "them");
+        Arena A;
+        A.base = scratch_address();
+        A.size = kScratchBytes;
+        // chunks of points whose work buffers fill the scratch
+        const int split = precision == GIMIC_PREC_FP64 ? kScratchMaxSplit
+                          : c->mixed_variant < 0       ? kDefaultSplit
+                          : c->mixed_split > 0         ? c->mixed_split
+                                                       : kScratchMaxSplit;
+        const size_t rows = scratch_rows(c, precision, 1, kScratchBytes, split);
+        if (rows == 0)
+            throw Error(GIMIC_ENOMEM, "the " + std::to_string(kScratchBytes >> 20) +
+                                          " MiB device scratch holds less than one row tile at " +
+                                          std::to_string(c->basis.nbf) + " basis functions");
+        long long refined = 0;
+        for (int64_t start = 0; start < npts; start += (int64_t)rows) {
+            const int64_t n = std::min<int64_t>((int64_t)rows, npts - start);
+            Workspace W;
+            A.reset();
+            evaluate(c, n, d_r + 3 * start, d_jt + 9 * start, spins, precision, st, nullptr,
+                     EvalMode::Scratch, nullptr, 0, EvalWork{&W, &A, rows});
+            if (refine) {
+                // the mixed pass's buffers are free once its launches are queued
+                Workspace W2;
+                A.reset();
+                refine_near_nuclei(c, n, d_r + 3 * start, d_jt + 9 * start, spins, st, nullptr,
+                                   EvalWork{&W2, &A, 0});
+                refined += c->refined_last;
+            }
+        }
+        c->refined_last = refined;
+    });
 }
 
 int gimic_jtensor_batch(gimic_ctx *c, int64_t npts, const double *r, double *jt, int spin,

@@ -19,7 +19,7 @@ EXPORTS = [
     "gimic_set_screening", "gimic_calc_jtensor", "gimic_calc_jvector", "gimic_calc_modj",
     "gimic_get_gauss_points", "mkgausspoints", "gimic_last_error", "gimic_ctx_create_files",
     "gimic_ctx_create", "gimic_ctx_set_density", "gimic_ctx_set_flags", "gimic_ctx_info",
-    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_atom_symbols", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_tuning_log", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_tile_stats", "gimic_ctx_tile_stats", "gimic_ctx_set_fp64_radius", "gimic_ctx_fp64_refined", "gimic_ctx_set_chunk_bytes", "gimic_ctx_chunk_bytes", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device",
+    "gimic_ctx_destroy", "gimic_ctx_atoms", "gimic_ctx_atom_symbols", "gimic_ctx_set_timing", "gimic_ctx_get_timing", "gimic_ctx_tuning", "gimic_ctx_tuning_log", "gimic_ctx_set_mixed_kernel", "gimic_ctx_set_tile_stats", "gimic_ctx_tile_stats", "gimic_ctx_set_fp64_radius", "gimic_ctx_fp64_refined", "gimic_ctx_set_chunk_bytes", "gimic_ctx_chunk_bytes", "gimic_ctx_set_graph", "gimic_ctx_graph_stats", "gimic_jtensor_batch", "gimic_jtensor_batch_device", "gimic_jtensor_batch_device_noalloc",
     "gimic_jtensor_multi_device", "gimic_flux_planes", "gimic_flux_plane_grids", "gimic_ctx_properties", "gimic_sph_transform", "gimic_quadrature",
     "gimic_selftest_mfma_layout", "gimic_selftest_mfma32_layout", "gimic_mlp_create", "gimic_mlp_destroy", "gimic_mlp_set_stream", "gimic_mlp_info",
     "gimic_mlp_params", "gimic_mlp_loss_grad", "gimic_mlp_train_step", "gimic_mlp_predict",
@@ -88,6 +88,7 @@ def load():
         "gimic_ctx_graph_stats": [P, P, P],
         "gimic_jtensor_batch": [P, I64, P, P, I, I],
         "gimic_jtensor_batch_device": [P, I64, P, P, I, I, P],
+        "gimic_jtensor_batch_device_noalloc": [P, I64, P, P, I, I, P],
         "gimic_jtensor_multi_device": [P, I64, P, P, I, P, I, P],
         "gimic_flux_planes": [P, I, P, P, P, P, P, P, P, I, I, P, P, P, P],
         "gimic_flux_plane_grids": [P, I, P, P, P, P, P, P, P, P, P, P, P, P, P, I, I, P, P, P],
@@ -110,7 +111,7 @@ def load():
         fn = getattr(L, name)
         fn.argtypes = args
         if name.startswith(("gimic_ctx", "gimic_mlp")) or name in ("gimic_jtensor_batch", "gimic_jtensor_batch_device",
-                                                   
+                                                    "gimic_jtensor_batch_device_noalloc", "gimic_jtensor_batch_device_noalloc",
                                                     "gimic_jtensor_multi_device", "gimic_flux_planes",
                                                     "gimic_flux_plane_grids",
                                                     "gimic_sph_transform", "gimic_quadrature",
@@ -389,6 +390,24 @@ class Context:
         check(self._lib.gimic_jtensor_batch_device(self._h, n, ctypes.c_void_p(pts.data_ptr()),
                                                    ctypes.c_void_p(out.data_ptr()), SPIN[spin],
                                                    PREC[precision], s))
+        return out
+
+    def jtensor_device_noalloc(self, points, spin="total", precision="fp64", out=None,
+                               stream=None):
+        """as jtensor_device through the allocation-free entry point (work
+        buffers in the library's per-device scratch array; the right operands
+        must be resident from an earlier jtensor_device call)"""
+        import torch
+        pts = points.contiguous()
+        assert pts.dtype == torch.float64 and pts.is_cuda
+        n = pts.shape[0]
+        if out is None:
+            out = torch.empty((n, 9), dtype=torch.float64, device=pts.device)
+        s = ctypes.c_void_p(stream) if stream is not None else \
+            ctypes.c_void_p(torch.cuda.current_stream(pts.device).cuda_stream)
+        check(self._lib.gimic_jtensor_batch_device_noalloc(
+            self._h, n, ctypes.c_void_p(pts.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+            SPIN[spin], PREC[precision], s))
         return out
 
     def jtensor_multi_device(self, points, spins, precision="fp64", out=None, stream=None):

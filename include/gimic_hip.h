@@ -216,6 +216,25 @@ int gimic_jtensor_batch(gimic_ctx *ctx, int64_t npts, const double *r, double *j
 int gimic_jtensor_batch_device(gimic_ctx *ctx, int64_t npts, const double *d_r, double *d_jt,
                                int spin, int precision, void *stream);
 
+/* Allocation-free variant of gimic_jtensor_batch_device (no counterpart in
+ * the reference; for callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, hosts that forbid it).
+ * Same arguments, results to the rounding of the precision.  The call
+ * allocates nothing: its work buffers live in a 256 MiB device array that
+ * the library holds on every GPU it runs on until the process ends, and it
+ * evaluates the points in chunks that fit there (any npts; a basis too wide
+ * for one 256-row chunk is rejected with GIMIC_ENOMEM).  The context's
+ * right operands for this spin case and precision must be resident: one
+ * gimic_jtensor_batch_device call with them (one point suffices) builds
+ * them, otherwise GIMIC_ESTATE.  An untuned context uses mixed kernel 9
+ * with 4 N-ranges and does not tune.  Every call on one GPU shares that
+ * GPU's array: calls that overlap in time on one GPU (any context, any
+ * stream) race and must be serialised by the caller.  With mixed precision
+ * the FP64 pass near nuclei synchronises the stream once per chunk, as in
+ * the allocating call. */
+int gimic_jtensor_batch_device_noalloc(gimic_ctx *ctx, int64_t npts, const double *d_r,
+                                       double *d_jt, int spin, int precision, void *stream);
+
 /* Several spin cases from one pass over the basis values: jt holds nspin
  * blocks of 9*npts doubles in the order of spins[].  Device pointers. */
 int gimic_jtensor_multi_device(gimic_ctx *ctx, int64_t npts, const double *d_r, double *d_jt,
