@@ -357,6 +357,16 @@ def bench_property(args):
 
 
 def bench_surrogate(args):
+    """The learned surrogate on the 60-atom cage, 256^3 visualisation grid
+    over [-7.68, 7.68]^3 bohr.  'stated': the workload as given -- exact
+    kernel within 0.4 bohr of every nucleus, 2M training points (uniform plus
+    Gaussian clouds of 1 bohr around the nuclei), 1% held out, a cell of the
+    8^3 served only where its held-out error is within 1e-3.  'far_field'
+    beside it: exact within 3 bohr, uniform samples, reported at looser
+    tolerances, where the network does serve.  Each line times the hybrid
+    evaluation (network on served points, exact mixed kernel on the rest)
+    against the exact kernel alone and audits the served points with the
+    exact FP64 kernel."""
     import torch
     from gimic_amd import synth
     from gimic_amd.surrogate import Surrogate
@@ -364,44 +374,44 @@ def bench_surrogate(args):
     ctx = system.context(screening=1e-8)
     ctx.set_chunk_bytes(-1)  # the large chunk: a third of device memory
     half = 7.68
-    # the network covers points farther than `exclusion` bohr from every
-    # nucleus (the field there is smooth enough to learn); nearer points and
-    # cells that fail the held-out tolerance go to the exact kernel
-    sg = Surrogate(ctx, np.asarray(system.coords), -half * np.ones(3), half * np.ones(3),
-                   tol=1e-3, ncell=8, sigma=args.sigma, seed=3, exclusion=args.exclusion)
-    st = sg.fit(n_train=2_000_000, steps=args.train_steps, batch=16384, lr=2e-3, near_frac=0.0)
-    st.pop("loss_history", None)
     ax = torch.linspace(-half, half, 256, dtype=torch.float64, device="cuda")
     z, y, x = torch.meshgrid(ax, ax, ax, indexing="ij")
     grid = torch.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    del x, y, z
     n = grid.shape[0]
     out = torch.empty((n, 9), dtype=torch.float64, device="cuda")
-    sub = grid[::8].contiguous()
-    ctx.jtensor_device(sub, precision="mixed")
+    ctx.jtensor_device(grid, precision="mixed", out=out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ctx.jtensor_device(sub, precision="mixed")
+    ctx.jtensor_device(grid, precision="mixed", out=out)
     torch.cuda.synchronize()
-    t_ex = (time.perf_counter() - t0) * 8  # exact-only time for the full grid
-    t0 = time.perf_counter()
-    sg.net.predict(grid, out=out)
-    torch.cuda.synchronize()
-    t_net = time.perf_counter() - t0
-    per_tol = {}
-    for tol in (1e-3, 1e-2, 3e-2):
-        sg.set_tolerance(tol)
-        sg.evaluate(grid, out=out)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        J, served = sg.evaluate(grid, out=out)
-        torch.cuda.synchronize()
-        t_h = time.perf_counter() - t0
-        per_tol[f"{tol:g}"] = {"served_fraction": round(float(served.float().mean()), 4),
-                               "points_per_s": round(n / t_h, 1),
-                               "gain_over_exact": round(t_ex / t_h, 3),
-                               "audit": sg.audit(grid, J, served)}
+    t_ex = time.perf_counter() - t0
+    runs = {}
+    for name, excl, near_frac in (("stated", 0.4, 0.5), ("far_field", 3.0, 0.0)):
+        log(f"surrogate {name}")
+        sg = Surrogate(ctx, np.asarray(system.coords), -half * np.ones(3), half * np.ones(3),
+                       tol=1e-3, ncell=8, sigma=args.sigma, seed=3, exclusion=excl)
+        st = sg.fit(n_train=2_000_000, steps=args.train_steps, batch=16384, lr=2e-3,
+                    near_frac=near_frac)
+        st.pop("loss_history", None)
+        per_tol = {}
+        for tol in (1e-3, 1e-2, 3e-2):
+            sg.set_tolerance(tol)
+            sg.evaluate(grid, out=out)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            J, served = sg.evaluate(grid, out=out)
+            torch.cuda.synchronize()
+            t_h = time.perf_counter() - t0
+            per_tol[f"{tol:g}"] = {"served_fraction": round(float(served.float().mean()), 4),
+                                   "points_per_s": round(n / t_h, 1),
+                                   "gain_over_exact": round(t_ex / t_h, 3),
+                                   "audit": sg.audit(grid, J, served)}
+        runs[name] = {"exclusion_bohr": excl, "near_frac": near_frac, "training": st,
+                      "by_tolerance": per_tol}
+        sg.net.close()
     ctx.close()
-    head = per_tol["0.001"]
+    head = runs["stated"]["by_tolerance"]["0.001"]
     return {
         "metric": "visualisation-grid J-tensor points/s with learned surrogate + exact fallback",
         "value": head["points_per_s"], "unit": "points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -409,12 +419,10 @@ def bench_surrogate(args):
         "dtype": "bf16 MFMA (fp32 master weights); fallback fp32 (bf16x6 MFMA)",
         "data": "synthetic",
         "config": {"workload": "cage60 (nbf 1800) on a 256^3 grid over [-7.68, 7.68]^3 bohr; "
-                               "2M exact-kernel training points beyond %.1f bohr of every "
-                               "nucleus, 1%% held out, 8^3 cells" % args.exclusion,
+                               "2M exact-kernel training points, 1% held out, 8^3 cells; "
+                               "tolerance 1e-3, exact within 0.4 bohr of every nucleus",
                    "points_per_step": n},
-        "tolerance_1e-3": head, "by_tolerance": per_tol,
-        "exact_only_points_per_s": round(n / t_ex, 1), "network_only_points_per_s": round(n / t_net, 1),
-        "training": st,
+        "tolerance_1e-3": head, "exact_only_points_per_s": round(n / t_ex, 1), "runs": runs,
     }
 
 
@@ -429,8 +437,6 @@ def main():
                     choices=["cage60", "scan", "openshell", "surrogate", "property"])
     ap.add_argument("--train-steps", type=int, default=12000, help="surrogate training steps")
     ap.add_argument("--sigma", type=float, default=1.0, help="surrogate Fourier frequency scale")
-    ap.add_argument("--exclusion", type=float, default=3.0,
-                    help="surrogate: exact kernel within this distance (bohr) of a nucleus")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the workload is sharded by point tiles "

@@ -269,13 +269,23 @@ class Surrogate:
         if self.net is None:
             raise RuntimeError("surrogate not trained")
         r = r.contiguous()
-        served = self.served_mask(r)
         if out is None:
             out = torch.empty((r.shape[0], 9), dtype=torch.float64, device=r.device)
-        self.net.predict(r, out=out)
+        if not bool(self.cell_ok.any()):
+            # no cell validated: the exact kernel everywhere
+            self.ctx.jtensor_device(r, spin=self.spin, precision=self.precision, out=out)
+            return out, torch.zeros(r.shape[0], dtype=torch.bool, device=r.device)
+        served = self.served_mask(r)
+        if bool(served.all()):
+            self.net.predict(r, out=out)
+            return out, served
+        # the network on the served points only, the exact kernel on the rest
+        sv = served.nonzero().squeeze(1)
+        if sv.numel():
+            out[sv] = self.net.predict(r[sv].contiguous())
         ex = (~served).nonzero().squeeze(1)
-        if ex.numel():
-            out[ex] = self.ctx.jtensor_device(r[ex], spin=self.spin, precision=self.precision)
+        out[ex] = self.ctx.jtensor_device(r[ex].contiguous(), spin=self.spin,
+                                          precision=self.precision)
         return out, served
 
     def audit(self, r, J, served, nsample=100_000, seed=7):

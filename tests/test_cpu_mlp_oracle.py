@@ -116,3 +116,19 @@ def test_features_periodicity(seed):
     x0 = M.features(r, f, cs)
     x1 = M.features(r + np.array([cs[3], 0, 0]), f, cs)
     assert np.abs(x0[:, :2 * M.F] - x1[:, :2 * M.F]).max() < 1e-12
+
+
+@pytest.mark.parametrize("kind,layer", [("no_dsilu", l) for l in range(1, M.D + 1)] +
+                         [("no_bias", l) for l in range(M.D + 1)])
+def test_gpu_gradient_bounds_catch_a_planted_mistake(kind, layer):
+    """a missing SiLU' or bias gradient in any one layer moves some weight or
+    bias block by far more than the per-block bound of the GPU gradient test
+    (so that test would fail on such a kernel)"""
+    p, freqs, cs, os_, rng = _net(seed=13, sigma=2.0)
+    r = rng.uniform(-6, 6, (512, 3))
+    t = rng.normal(0, 1, (512, 9))
+    _, g = M.loss_grad(p, freqs, cs, os_, r, t, 0.01, bf=True)
+    _, gp = M.loss_grad(p, freqs, cs, os_, r, t, 0.01, bf=True, plant=(kind, layer))
+    dev = max(np.linalg.norm(gp[lo:hi] - g[lo:hi]) / np.linalg.norm(g[lo:hi])
+              for _, (lo, hi) in M.blocks())
+    assert dev > 50 * M.GPU_BLOCK_TOL

@@ -9,6 +9,7 @@ from oracle import mlp_ref as M
 pytestmark = pytest.mark.gpu
 
 CS = np.array([0.0, 0.0, 0.0, 6.0])
+GRAD_TOL, BLOCK_TOL = M.GPU_GRAD_TOL, M.GPU_BLOCK_TOL
 
 
 def _setup(n=4096, seed=0):
@@ -64,14 +65,16 @@ def test_gradient_matches_oracle():
     l_64, g_64 = M.loss_grad(p, fr, CS, rms, rn, Jn, eps)
     print("loss gpu", loss, "bf16 oracle", l_emu, "fp64 oracle", l_64)
     print("grad rel err vs bf16 oracle", _rel(grad, g_emu), "vs fp64 oracle", _rel(grad, g_64))
-    assert abs(loss - l_emu) <= 1e-3 * l_emu
-    assert _rel(grad, g_emu) < 1e-2
+    assert abs(loss - l_emu) <= 1e-4 * l_emu
+    assert _rel(grad, g_emu) < GRAD_TOL
     assert _rel(grad, g_64) < 5e-2
-    # per layer too, so no layer can hide behind a larger one
-    lay, _ = M.layout()
-    for w, b, kin, nout in lay:
-        assert _rel(grad[w:b], g_emu[w:b]) < 2e-2
-        assert _rel(grad[b:b + nout], g_emu[b:b + nout]) < 2e-2
+    # per weight and bias block too, so no layer can hide behind a larger
+    # one: a missing SiLU' or bias term moves its block by 0.8-1.0
+    # (test_cpu_mlp_oracle.py plants both in every layer)
+    for name, (lo, hi) in M.blocks():
+        e = _rel(grad[lo:hi], g_emu[lo:hi])
+        print("block", name, "rel err %.2e" % e)
+        assert e < BLOCK_TOL, (name, e)
     net.close()
     ctx.close()
 
@@ -179,4 +182,48 @@ def test_surrogate_gains_over_exact_kernel():
     assert frac > 0.05
     assert audit["rel_err"] <= 5e-2
     assert t_hybrid < t_exact
+    ctx.close()
+
+
+def test_surrogate_at_stated_tolerance():
+    """the workload as stated: 2M training points (half uniform, half in
+    1-bohr clouds around the nuclei), exact kernel within 0.4 bohr, 8^3
+    cells, held-out tolerance 1e-3.  Whatever the network reaches, the
+    hybrid field must keep the contract: a cell is served exactly when its
+    held-out error is within the tolerance on >= 8 held-out points, the
+    served set only grows with the tolerance, unserved points carry the
+    exact kernel's tensors, and served points meet the tolerance when the
+    FP64 kernel judges them"""
+    import torch
+    from gimic_amd import synth
+    from gimic_amd.surrogate import Surrogate
+    s = synth.cage60()
+    ctx = s.context()
+    half = 7.68
+    sg = Surrogate(ctx, np.asarray(s.coords), -half * np.ones(3), half * np.ones(3), tol=1e-3,
+                   ncell=8, sigma=1.0, seed=3, exclusion=0.4)
+    print("case surrogate stated tolerance", flush=True)
+    st = sg.fit(n_train=2_000_000, steps=3000, batch=16384, lr=2e-3, near_frac=0.5)
+    assert st["n_holdout"] == 20000
+    err, cnt = sg._cell_err, sg._cell_cnt
+    assert torch.equal(sg.cell_ok, (err <= 1e-3) & (cnt >= 8))
+    ok3 = sg.cell_ok.clone()
+    sg.set_tolerance(3e-2)
+    assert bool((sg.cell_ok | ~ok3).all())           # served set grows with the tolerance
+    sg.set_tolerance(1e-3)
+    assert torch.equal(sg.cell_ok, ok3)
+    ax = torch.linspace(-half, half, 48, dtype=torch.float64, device="cuda")
+    z, y, x = torch.meshgrid(ax, ax, ax, indexing="ij")
+    grid = torch.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    J, served = sg.evaluate(grid)
+    ex = ctx.jtensor_device(grid, precision="mixed")
+    un = ~served
+    print("held-out rel err %.3e" % st["holdout_rel_err"], "cells served", st["cells_served"],
+          "grid served %.4f" % float(served.float().mean()), flush=True)
+    assert torch.allclose(J[un], ex[un], rtol=1e-12, atol=1e-15 * float(ex.abs().max()))
+    audit = sg.audit(grid, J, served)
+    print("audit", audit)
+    if audit["sampled"]:
+        assert audit["rel_err"] <= 3e-3
+    sg.net.close()
     ctx.close()
