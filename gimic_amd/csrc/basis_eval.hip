@@ -392,7 +392,11 @@ void basis_eval_staged_kernel(EvalArgs a) {
     const int ncand1 = kCand && one_chunk ? candidates(0) : 0;
     // rows per pass: as many as the LDS image holds at this width (at least
     // one: the image holds a dense row)
-    const int rpp = max(1, min(tp, (int)(a.lds_image / ((size_t)wt * kFunBytes))));
+    // rounded down to a power of two, which divides tp (1, 2, 4 or 8): the
+    // pair index then splits into row and shell by a shift and a mask
+    int rpp = max(1, min(tp, (int)(a.lds_image / ((size_t)wt * kFunBytes))));
+    while (rpp & (rpp - 1)) rpp &= rpp - 1;
+    const int rsh = __builtin_ctz(rpp);
     for (int r0 = 0; r0 < tp; r0 += rpp) {
         const int nr = min(rpp, tp - r0);
         EvalArgs b = a;
@@ -436,11 +440,12 @@ void basis_eval_staged_kernel(EvalArgs a) {
         if constexpr (kCand) {
             for (int s0 = 0; s0 < a.nshell; s0 += kCandChunk) {
                 const int ncand = one_chunk ? ncand1 : candidates(s0);
-                for (int it = tid; it < nr * ncand; it += 256) pair(it % nr, s0 + cand[it / nr]);
+                for (int it = tid; it < (ncand << rsh); it += 256)
+                    pair(it & (rpp - 1), s0 + cand[it >> rsh]);
                 if (!one_chunk) __syncthreads();  // cand[] is read out before the next chunk refills it
             }
         } else {
-            for (int it = tid; it < nr * a.nshell; it += 256) pair(it % nr, it / nr);
+            for (int it = tid; it < (a.nshell << rsh); it += 256) pair(it & (rpp - 1), it >> rsh);
         }
         __syncthreads();
         // this pass's rows are contiguous in both outputs
