@@ -21,6 +21,11 @@ modes.  All integrals come from one tensor evaluation per spin case (the
 reference evaluates J once per integral).
 
     python -m gimic_amd.driver gimic.inp [--precision fp64|mixed]
+    python -m gimic_amd.driver - < keyword-stream     (as gimic.bin is run)
+
+The input may be a gimic.inp or the keyword stream the reference's frontend
+pipes into gimic.bin (`-` reads it from standard input; MOL, XDENS and the
+output files are then relative to the working directory).
 """
 import argparse
 import os
@@ -209,11 +214,16 @@ def run_property(ctx, basedir=".", spin="total", precision="fp64", out=sys.stdou
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("input", nargs="?", default="gimic.inp")
+    ap.add_argument("input", nargs="?", default="gimic.inp",
+                    help="gimic.inp or keyword stream; - reads the stream from standard input")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     a = ap.parse_args(argv)
-    kw = inp.parse(open(a.input).read())
-    kw, ctx, g, b, uhf = setup(kw, os.path.dirname(os.path.abspath(a.input)))
+    if a.input == "-":
+        text, basedir = sys.stdin.read(), os.getcwd()
+    else:
+        text, basedir = open(a.input).read(), os.path.dirname(os.path.abspath(a.input))
+    kw = inp.parse_any(text)
+    kw, ctx, g, b, uhf = setup(kw, basedir)
     coords, _ = ctx.atoms()
     writers.write_grid_xyz("grid.xyz", g, ctx.symbols(), coords)
     if kw.get("calc") == "integral":
@@ -221,7 +231,7 @@ def main(argv=None):
     else:
         run_cdens(kw, ctx, g, b, uhf, a.precision, outdir=".")
         if kw["Essential"].get("prop", False):
-            run_property(ctx, os.path.dirname(os.path.abspath(a.input)), precision=a.precision)
+            run_property(ctx, basedir, precision=a.precision)
     ctx.close()
 
 

@@ -8,6 +8,11 @@
 Returns a dict: top-level keywords plus one dict per section; a section's
 argument is stored under '_arg'.  Only keywords present in the file are
 returned; `with_defaults` adds the schema defaults of src/gimic.in.
+
+`parse_stream` reads the keyword stream that gimic.bin takes on its
+standard input (the reference's frontend parses gimic.inp against its
+template and pipes the result; getkw.f90:1526-1646 reads it) into the same
+dict; `parse_any` takes either form.
 """
 import re
 
@@ -91,3 +96,81 @@ def with_defaults(kw):
         else:
             out[k] = v
     return out
+
+
+_STREAM_SCALAR = {"INT": int, "DBL": lambda t: float(t.replace("d", "e").replace("D", "E")),
+                  "BOOL": lambda t: t.strip().lower() in ("t", "true", ".true.", "1", "yes", "on"),
+                  "STR": str}
+
+
+def parse_stream(text):
+    """Keyword stream (getkw.f90:1526-1646; written by the frontend's
+    getkw.py, Section/Keyword __str__) -> the dict of parse().
+
+    Section:  SECT <name> <nsect> <set>
+              ARG T|F KW <nkw>
+              [the section's argument, as a keyword]
+              <nkw keywords> <nsect sections>
+    Keyword:  <TYPE> <name> <n> <set>, then n value lines; n = 0: no value
+              (left out of the dict), STR with n = -1: the empty string.
+    """
+    lines = text.splitlines()
+    pos = 0
+
+    def line():
+        nonlocal pos
+        if pos >= len(lines):
+            raise ValueError("keyword stream: unexpected end")
+        pos += 1
+        return lines[pos - 1]
+
+    def keyword():
+        f = line().split()
+        if len(f) < 4:
+            raise ValueError(f"keyword stream: bad keyword header {' '.join(f)!r}")
+        typ, name, n = f[0], f[1], int(f[2])
+        base = typ[:-6] if typ.endswith("_ARRAY") else typ
+        if base not in _STREAM_SCALAR:
+            raise ValueError(f"keyword stream: invalid type {typ} for {name}")
+        if typ == "STR" and n == -1:
+            return name, ""
+        if n <= 0:
+            return name, None
+        if base == "STR":  # one record per value, read whole
+            vals = [line() for _ in range(n)]
+        else:  # list-directed: n values, possibly over several records
+            vals = []
+            while len(vals) < n:
+                vals += line().replace(",", " ").split()
+            vals = [_STREAM_SCALAR[base](v) for v in vals[:n]]
+        return name, (vals if typ.endswith("_ARRAY") or n > 1 else vals[0])
+
+    def section():
+        f = line().split()
+        if len(f) < 3 or f[0] != "SECT":
+            raise ValueError(f"keyword stream: expected SECT, got {' '.join(f)!r}")
+        name, nsect = f[1], int(f[2])
+        g = line().split()
+        if len(g) < 4 or g[0] != "ARG" or g[2] != "KW":
+            raise ValueError(f"keyword stream: expected ARG ... KW, got {' '.join(g)!r}")
+        d = {}
+        if g[1].upper().startswith("T"):
+            _, v = keyword()
+            if v is not None:
+                d["_arg"] = v
+        for _ in range(int(g[3])):
+            k, v = keyword()
+            if v is not None:
+                d[k] = v
+        for _ in range(nsect):
+            k, v = section()
+            d[k] = v
+        return name, d
+
+    _, top = section()
+    return top
+
+
+def parse_any(text):
+    """gimic.inp text or the keyword stream"""
+    return parse_stream(text) if text.lstrip().startswith("SECT ") else parse(text)

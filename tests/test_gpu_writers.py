@@ -94,3 +94,60 @@ def test_driver_writes_readgrid_vtu(tmp_path, monkeypatch):
     ref = writer_fixture("c4h4/read-grid/reference/jvec.vtu")
     assert runtest_mismatches(out, ref, 1e-8) == []
     assert (tmp_path / "grid.xyz").exists()
+
+
+# READGRID_INP as the reference's frontend pipes it into gimic.bin
+READGRID_STREAM = """SECT top 2 True
+ARG F KW 5
+STR calc 1 True
+cdens
+STR basis 1 True
+MOL
+STR xdens 1 True
+XDENS
+BOOL openshell 1 True
+False
+DBL_ARRAY magnet 3 True
+0.0
+0.0
+-1.0
+SECT Grid 0 True
+ARG T KW 1
+STR Grid 1 True
+file
+STR file 1 True
+gridfile.grd
+SECT Advanced 0 True
+ARG F KW 6
+BOOL spherical 1 True
+False
+BOOL diamag 1 True
+True
+BOOL paramag 1 True
+True
+BOOL GIAO 1 True
+True
+BOOL screening 1 True
+True
+DBL screening_thrs 1 True
+1.d-8
+"""
+
+
+def test_driver_reads_keyword_stream_on_stdin(tmp_path, monkeypatch):
+    """`python -m gimic_amd.driver - < stream`, as gimic.bin is run: the same
+    jvec.vtu as from the gimic.inp"""
+    import io
+    from gimic_amd import inp
+    assert inp.parse_stream(READGRID_STREAM) == inp.parse(READGRID_INP)
+    write_fixture(str(tmp_path), "c4h4.npz")
+    d = golden("c4h4.npz")
+    np.savetxt(tmp_path / "gridfile.grd", d["readgrid_points"], fmt="%.17g")
+    (tmp_path / "grid.1.ele").write_text(writer_fixture("c4h4/read-grid/grid.1.ele"))
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setattr("sys.stdin", io.StringIO(READGRID_STREAM))
+    print("case driver c4h4/read-grid from the keyword stream", flush=True)
+    driver.main(["-"])
+    out = (tmp_path / "jvec.vtu").read_text()
+    ref = writer_fixture("c4h4/read-grid/reference/jvec.vtu")
+    assert runtest_mismatches(out, ref, 1e-8) == []
