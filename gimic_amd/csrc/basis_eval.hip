@@ -66,6 +66,10 @@ __device__ __forceinline__ void radial_parts(const EvalArgs &a, const DevShell &
     }
 }
 
+// Candidate lists (below) for bases of at least this many shells: with fewer,
+// nearly every shell reaches every tile (tetracene, 144 shells: 6% slower
+// with the list, which costs registers and a pass over the shells)
+constexpr int kCandMinShells = 400;
 // Shells whose candidacy the staged kernel settles per pass of its loop
 constexpr int kCandChunk = 1024;
 // LDS beyond the image and the block positions: candidate list, bounding
@@ -291,7 +295,7 @@ __global__ __launch_bounds__(256) void basis_eval_kernel(EvalArgs a) {
 // wavefront's shells (ordered by l) need not own adjacent columns.
 // (four wavefronts per SIMD for l <= 3: the LDS images of four workgroups
 // fit a CU, and the registers are held to 128)
-template <int LMAX, bool F64 = false>
+template <int LMAX, bool F64 = false, bool CAND = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LMAX <= 3 ? 4 : 1)))
 void basis_eval_staged_kernel(EvalArgs a) {
     constexpr int kFunBytes = F64 ? 32 : 2 * kAParts + 12;  // image bytes per function
@@ -317,7 +321,7 @@ void basis_eval_staged_kernel(EvalArgs a) {
         grad_dst = a.grad32 + base * 3 + rin * (size_t)wt;
         grad_plane = (size_t)a.tile_rows * wt;
     }
-    // Candidate shells: the shells that can reach a row of this workgroup,
+    // Candidate shells (CAND): the shells that can reach a row of this workgroup,
     // listed in shell order, kCandChunk at a time.  A shell is dropped only
     // if its block is not listed (compressed output) or if it is beyond its
     // screening radius from the rows' bounding sphere, hence from every row;
@@ -327,7 +331,7 @@ void basis_eval_staged_kernel(EvalArgs a) {
     short *cand = (short *)(basis_tile + a.lds_image + ((size_t)a.nblk * 2 + 15) / 16 * 16);
     double *sphere = (double *)(cand + kCandChunk);  // centre x, y, z and radius
     int *wcnt = (int *)(sphere + 4);                 // kept shells per wavefront
-    if (!F64) {
+    if (CAND) {
         if (tid == 0) {
             double c[3] = {0.0, 0.0, 0.0}, rad = -1.0;
             int nv = 0;
@@ -387,16 +391,12 @@ void basis_eval_staged_kernel(EvalArgs a) {
     // for all passes.  The FP64 rows (one or two per workgroup) keep the
     // plain walk over every shell: for so few rows the list costs more than
     // the idle lanes it saves (8.2 ms against 9.7 ms on the 60-atom cage)
-    constexpr bool kCand = !F64;
+    constexpr bool kCand = CAND && !F64;
     const bool one_chunk = !kCand || a.nshell <= kCandChunk;
     const int ncand1 = kCand && one_chunk ? candidates(0) : 0;
     // rows per pass: as many as the LDS image holds at this width (at least
     // one: the image holds a dense row)
-    // rounded down to a power of two, which divides tp (1, 2, 4 or 8): the
-    // pair index then splits into row and shell by a shift and a mask
-    int rpp = max(1, min(tp, (int)(a.lds_image / ((size_t)wt * kFunBytes))));
-    while (rpp & (rpp - 1)) rpp &= rpp - 1;
-    const int rsh = __builtin_ctz(rpp);
+    const int rpp = max(1, min(tp, (int)(a.lds_image / ((size_t)wt * kFunBytes))));
     for (int r0 = 0; r0 < tp; r0 += rpp) {
         const int nr = min(rpp, tp - r0);
         EvalArgs b = a;
@@ -440,12 +440,11 @@ void basis_eval_staged_kernel(EvalArgs a) {
         if constexpr (kCand) {
             for (int s0 = 0; s0 < a.nshell; s0 += kCandChunk) {
                 const int ncand = one_chunk ? ncand1 : candidates(s0);
-                for (int it = tid; it < (ncand << rsh); it += 256)
-                    pair(it & (rpp - 1), s0 + cand[it >> rsh]);
+                for (int it = tid; it < nr * ncand; it += 256) pair(it % nr, s0 + cand[it / nr]);
                 if (!one_chunk) __syncthreads();  // cand[] is read out before the next chunk refills it
             }
         } else {
-            for (int it = tid; it < (a.nshell << rsh); it += 256) pair(it & (rpp - 1), it >> rsh);
+            for (int it = tid; it < nr * a.nshell; it += 256) pair(it % nr, it / nr);
         }
         __syncthreads();
         // this pass's rows are contiguous in both outputs
@@ -496,7 +495,7 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
         b.lds_image = tp * row_bytes;
         b.toff = nullptr;
         b.nblk = 0;  // no block positions: the candidate list follows the image
-        const size_t lds = b.lds_image + kCandBytes;
+        const size_t lds = b.lds_image;
         auto launch = [&](auto kernel) -> hipError_t {
             hipError_t e = hipFuncSetAttribute((const void *)kernel,
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -530,8 +529,10 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
         if (row_bytes > 150 * 1024) return hipErrorInvalidValue;
         b.tile_points = tp;
         b.lds_image = image;
-        // the image, the block positions (16-byte aligned) and the candidate list
-        const size_t lds = image + ((size_t)a.nblk * 2 + 15) / 16 * 16 + kCandBytes;
+        // the image, the block positions (16-byte aligned) and, for bases of
+        // kCandMinShells shells or more, the candidate list
+        const bool cand = a.nshell >= kCandMinShells;
+        const size_t lds = image + ((size_t)a.nblk * 2 + 15) / 16 * 16 + (cand ? kCandBytes : 0);
         if (a.rows % tp || a.lmax < 0 || a.lmax > kDevMaxL) return hipErrorInvalidValue;
         if (a.toff && (a.tile_rows % tp || a.rows % a.tile_rows || a.nblk * 32 != a.lda))
             return hipErrorInvalidValue;
@@ -544,10 +545,17 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
             return hipGetLastError();
         };
         hipError_t e;
-        if (a.lmax <= 1) e = launch(basis_eval_staged_kernel<1>);
-        else if (a.lmax == 2) e = launch(basis_eval_staged_kernel<2>);
-        else if (a.lmax == 3) e = launch(basis_eval_staged_kernel<3>);
-        else e = launch(basis_eval_staged_kernel<5>);
+        if (cand) {
+            if (a.lmax <= 1) e = launch(basis_eval_staged_kernel<1, false, true>);
+            else if (a.lmax == 2) e = launch(basis_eval_staged_kernel<2, false, true>);
+            else if (a.lmax == 3) e = launch(basis_eval_staged_kernel<3, false, true>);
+            else e = launch(basis_eval_staged_kernel<5, false, true>);
+        } else {
+            if (a.lmax <= 1) e = launch(basis_eval_staged_kernel<1>);
+            else if (a.lmax == 2) e = launch(basis_eval_staged_kernel<2>);
+            else if (a.lmax == 3) e = launch(basis_eval_staged_kernel<3>);
+            else e = launch(basis_eval_staged_kernel<5>);
+        }
         if (e != hipSuccess) return e;
     }
     return hipGetLastError();
