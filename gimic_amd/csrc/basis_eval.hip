@@ -16,7 +16,7 @@
 // The split output of the mixed kernels (phi as three BF16 parts plus FP32
 // gradients), and the FP64 rows of bases up to 4,800 functions, are
 // produced by basis_eval_staged_kernel: a workgroup
-// owns a tile of 1-8 whole point rows, its threads evaluate (point,
+// owns a tile of 1-32 whole point rows, its threads evaluate (point,
 // contraction) pairs into a zeroed LDS image of those rows, and the rows
 // then leave with 16-byte coalesced stores.  Written directly, every store
 // instruction of a wavefront scatters 64 short values over a whole row.
@@ -278,7 +278,8 @@ __global__ __launch_bounds__(256) void basis_eval_kernel(EvalArgs a) {
 }
 
 // Split output through LDS (see the file comment).  Grid: rows / tile_points
-// workgroups; tile_points divides 128 and the LDS image is
+// workgroups; tile_points divides 128 (the smallest row tile of the
+// contraction kernels) and the LDS image is
 // tile_points * lda * 18 bytes (then, compressed, nblk shorts).
 //
 // Compressed output (a.toff non-null): the workgroup's rows belong to row
@@ -332,29 +333,35 @@ void basis_eval_staged_kernel(EvalArgs a) {
     double *sphere = (double *)(cand + kCandChunk);  // centre x, y, z and radius
     int *wcnt = (int *)(sphere + 4);                 // kept shells per wavefront
     if (CAND) {
-        if (tid == 0) {
-            double c[3] = {0.0, 0.0, 0.0}, rad = -1.0;
-            int nv = 0;
-            for (int i = 0; i < tp; i++)
-                if (row0 + i < (size_t)a.npts) {
-                    for (int d = 0; d < 3; d++) c[d] += a.pts[3 * (row0 + i) + d];
-                    nv++;
-                }
-            if (nv) {
-                for (int d = 0; d < 3; d++) c[d] /= nv;
-                rad = 0.0;
-                for (int i = 0; i < tp; i++)
-                    if (row0 + i < (size_t)a.npts) {
-                        double q = 0.0;
-                        for (int d = 0; d < 3; d++) {
-                            const double t = a.pts[3 * (row0 + i) + d] - c[d];
-                            q += t * t;
-                        }
-                        rad = fmax(rad, sqrt(q));
-                    }
+        // the rows' bounding sphere: centre = mean of the valid rows, radius
+        // = the largest distance from it (wavefront 0, one row per lane)
+        if (tid < 64) {
+            const bool v = tid < tp && row0 + tid < (size_t)a.npts;
+            double p[3], c[3], n = v ? 1.0 : 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) c[d] = p[d] = v ? a.pts[3 * (row0 + tid) + d] : 0.0;
+#pragma unroll
+            for (int off = 32; off; off >>= 1) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) c[d] += __shfl_xor(c[d], off);
+                n += __shfl_xor(n, off);
             }
-            for (int d = 0; d < 3; d++) sphere[d] = c[d];
-            sphere[3] = rad;  // negative: no valid row
+            double r = 0.0;
+            if (v) {
+                double q = 0.0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const double t = p[d] - c[d] / n;
+                    q += t * t;
+                }
+                r = sqrt(q);
+            }
+#pragma unroll
+            for (int off = 32; off; off >>= 1) r = fmax(r, __shfl_xor(r, off));
+            if (tid == 0) {
+                for (int d = 0; d < 3; d++) sphere[d] = n > 0.0 ? c[d] / n : 0.0;
+                sphere[3] = n > 0.0 ? r : -1.0;  // negative: no valid row
+            }
         }
         __syncthreads();
     }
@@ -513,12 +520,15 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
         hipLaunchKernelGGL(basis_eval_kernel<0>, dim3(nblk), dim3(256), 0, s, a);
     else {
         // dense: the largest tile of 1, 2, 4 or 8 rows whose LDS image fits in
-        // 72 KiB (two workgroups per CU).  Compressed: 8 rows per workgroup
+        // 72 KiB (two workgroups per CU).  Compressed: 32 rows per workgroup
         // staged in passes through an image of 36 KiB or one dense row,
-        // whichever is larger (four workgroups per CU at nbf <= 2048)
+        // whichever is larger (four workgroups per CU at nbf <= 2048); 32
+        // rows rather than 8 spread the workgroup's fixed work (block
+        // positions, candidate list) over more rows: 6-8% on the cage's and
+        // the tetracene scan's basis evaluation, 64 rows no better
         EvalArgs b = a;
         const size_t row_bytes = (size_t)a.lda * (2 * kAParts + 12);
-        int tp = 8;
+        int tp = a.toff ? 32 : 8;
         size_t image;
         if (a.toff) {
             image = std::max<size_t>(36 * 1024, row_bytes);
@@ -533,7 +543,8 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
         // kCandMinShells shells or more, the candidate list
         const bool cand = a.nshell >= kCandMinShells;
         const size_t lds = image + ((size_t)a.nblk * 2 + 15) / 16 * 16 + (cand ? kCandBytes : 0);
-        if (a.rows % tp || a.lmax < 0 || a.lmax > kDevMaxL) return hipErrorInvalidValue;
+        // (the candidate list's bounding sphere takes one row per lane of a wavefront)
+        if (a.rows % tp || tp > 64 || a.lmax < 0 || a.lmax > kDevMaxL) return hipErrorInvalidValue;
         if (a.toff && (a.tile_rows % tp || a.rows % a.tile_rows || a.nblk * 32 != a.lda))
             return hipErrorInvalidValue;
         auto launch = [&](auto kernel) -> hipError_t {
