@@ -24,7 +24,6 @@ Other workloads (one JSON line each; not the headline):
 import argparse
 import json
 import os
-import subprocess
 import sys
 import tempfile
 import time

@@ -7,7 +7,6 @@
   index would break
 """
 import math
-import os
 
 import numpy as np
 import pytest

@@ -5,13 +5,11 @@ Tolerances: FP64 path <= 1e-10 relative per point (error floor 1e-3 of the
 largest tensor in the set); the mixed path (FP32 operands split into three BF16 parts) <= 1e-5 per point
 with the same floor.
 """
-import ctypes
-import os
 
 import numpy as np
 import pytest
 
-from conftest import golden, rel_err, write_fixture
+from conftest import golden, rel_err
 from oracle import molio, oracle
 
 pytestmark = pytest.mark.gpu

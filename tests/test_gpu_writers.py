@@ -10,7 +10,6 @@ directory) with the reference's own test inputs:
   * test/c4h4/read-grid (file grid, B = (0, 0, -1)): jvec.vtu at rel 1e-8;
   * grid.xyz in both runs.
 """
-import os
 
 import numpy as np
 import pytest
