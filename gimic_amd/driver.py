@@ -16,9 +16,10 @@ and runs
                  with Essential.prop, the shielding of every nucleus and the
                  magnetizability on the external numgrid grid in coord.au,
                  gridfile.grd, grid_w.grd and nelpts.info (jfield.f90:584-929)
-printing the reference's stdout blocks; grid.xyz (plot_grid_xyz) in both
-modes.  All integrals come from one tensor evaluation per spin case (the
-reference evaluates J once per integral).
+printing the reference's stdout blocks (the grid and field set-up lines
+first); mol.xyz (write_xyz) and grid.xyz (plot_grid_xyz) in both modes.
+All integrals come from one tensor evaluation per spin case (the reference
+evaluates J once per integral).
 
     python -m gimic_amd.driver gimic.inp [--precision fp64|mixed]
     python -m gimic_amd.driver - < keyword-stream     (as gimic.bin is run)
@@ -51,15 +52,72 @@ def setup(kw, basedir="."):
         gkw["file"] = path(gkw["file"])
     magnet = kw.get("magnet") if "magnet" in kw and not kw.get("magnet_axis") else None
     g = gridmod.new_grid(gkw, coords, magnet=magnet)
-    b = gridmod.get_magnet(g, kw.get("magnet_axis", ""), kw.get("magnet"))
+    g.magnet_log = []
+    b = gridmod.get_magnet(g, kw.get("magnet_axis", ""), kw.get("magnet"), log=g.magnet_log)
     return kw, ctx, g, b, uhf
+
+
+def setup_report(g, uhf, out=None):
+    """what gimic.F90:159-171 prints between the density and the
+    calculation: the grid's set-up, the grid plot note, the field and the
+    shell type (to `out`, standard output when None)"""
+    out = sys.stdout if out is None else out
+    for line in g.log + [" *** Grid plot in grid.xyz"] + g.magnet_log:
+        print(line, file=out)
+    print(" INFO: Open-shell calculation" if uhf else " INFO: Closed-shell calculation", file=out)
+    print("", file=out)
 
 
 def spin_cases(uhf):
     return ["total", "alpha", "beta", "spindens"] if uhf else ["total"]
 
 
-def run_integral(kw, ctx, g, b, uhf, precision="fp64", out=sys.stdout):
+SPIN_NAME = {"total": "total", "alpha": "alpha", "beta": "beta", "spindens": "spin"}
+
+
+def integral_report(res, uhf, jmod, acid, radius, magnet_log=()):
+    """the printout of an integral run (run_integral, gimic.F90:222-261):
+    per spin case the |J| flux (when jmod), then the current flux, each
+    integration preceded by its spin note (open shell), the field check that
+    its get_magnet call prints (integral.f90:84, 224) and the radius bound
+    when one is set; then ACID for the total density (integrate_acid,
+    integral.f90:416-510).  res: {spin: {"modulus": (net, pos, neg),
+    "current": (net, pos, neg), "acid": value}}"""
+    spins = spin_cases(uhf)
+    bound = []
+    if radius < 1.0e10:
+        bound = [" " + writers.list_record("Integration bound set to radius ", float(radius))
+                 .rstrip()]
+
+    def note(spin):
+        return [f" *** Integrating {SPIN_NAME[spin]} density"] if uhf else []
+
+    lines = [" Integrating current density", " " + "*" * 41]
+    if jmod:
+        lines.append(" *** Integrating |J|")
+        for spin in spins:
+            lines += note(spin) + list(magnet_log) + bound
+            lines += flux.report(*res[spin]["modulus"], modulus=True)
+        lines.append("")
+    else:
+        lines.append(" Jmod integration skipped.")
+    lines.append(" *** Integrating current")
+    for spin in spins:
+        lines += note(spin) + list(magnet_log) + bound
+        lines += flux.report(*res[spin]["current"])
+    lines.append("")
+    if acid:
+        a = res["total"]["acid"]
+        lines += [" *** Integrating ACID density"] + note("total") + bound
+        lines += ["", " " + "*" * 60,
+                  " " + "   ACID (au) sqrt(delta J^2):%13.6f" % a,
+                  " " + "   ACID (nA/T)              :%13.6f" % flux.au2si(a),
+                  "", " " + "*" * 60, "", ""]
+    return lines
+
+
+def run_integral(kw, ctx, g, b, uhf, precision="fp64", out=None):
+    out = sys.stdout if out is None else out
     jmod = bool(kw["Essential"].get("jmod", False))
     # product grids are generated on the device from their 1-D rules (the
     # same points and weights, bit for bit); a file grid goes as host arrays
@@ -74,28 +132,9 @@ def run_integral(kw, ctx, g, b, uhf, precision="fp64", out=sys.stdout):
                                              modulus=True, acid=True)
         res[spin] = dict(current=tuple(cur[0]), modulus=tuple(mod[0]),
                          acid=float(np.sqrt(acid[0])))
-    if jmod:
-        print("*** Integrating |J|", file=out)
-        for spin in spin_cases(uhf):
-            if uhf:
-                print(f"*** Integrating {spin if spin != 'spindens' else 'spin'} density", file=out)
-            net, pos, neg = res[spin]["modulus"]
-            print(flux.report(net, pos, neg).replace("   Induced current (au)    :",
-                                                      "Induced mod current (au)   :")
-                  .replace("   Induced current (nA/T)  :", "Induced mod current (nA/T) :"),
-                  file=out)
-    print("*** Integrating current", file=out)
-    for spin in spin_cases(uhf):
-        if uhf:
-            print(f"*** Integrating {spin if spin != 'spindens' else 'spin'} density", file=out)
-        print(flux.report(*res[spin]["current"]), file=out)
-    if kw["Essential"].get("acid", False):
-        # integral.f90:500-510 (the reference integrates ACID for the total density)
-        a = res["total"]["acid"]
-        print("*" * 60, file=out)
-        print("   ACID (au) sqrt(delta J^2):%13.6f" % a, file=out)
-        print("   ACID (nA/T)              :%13.6f" % flux.au2si(a), file=out)
-        print("*" * 60, file=out)
+    for line in integral_report(res, uhf, jmod, bool(kw["Essential"].get("acid", False)),
+                                g.radius, getattr(g, "magnet_log", [])):
+        print(line, file=out)
     return res
 
 
@@ -150,10 +189,12 @@ def read_property_files(basedir="."):
     return coords, pts, w, sizes
 
 
-def run_property(ctx, basedir=".", spin="total", precision="fp64", out=sys.stdout):
+def run_property(ctx, basedir=".", spin="total", precision="fp64", out=None):
     """get_property (jfield.f90:584-929): shielding of every nucleus (ppm)
     and the magnetizability (au and SI) with positive/negative parts and
-    per-atom-block contributions, in the reference's print layout"""
+    per-atom-block contributions, in the reference's print layout (to `out`,
+    standard output when None)"""
+    out = sys.stdout if out is None else out
     files = read_property_files(basedir)
     if files is None:
         print(" at least one of the files coord.au, gridfile.grd, and grid_w.grd is missing."
@@ -225,7 +266,9 @@ def main(argv=None):
     kw = inp.parse_any(text)
     kw, ctx, g, b, uhf = setup(kw, basedir)
     coords, _ = ctx.atoms()
+    writers.write_mol_xyz("mol.xyz", ctx.symbols(), coords)
     writers.write_grid_xyz("grid.xyz", g, ctx.symbols(), coords)
+    setup_report(g, uhf)
     if kw.get("calc") == "integral":
         run_integral(kw, ctx, g, b, uhf, a.precision)
     else:

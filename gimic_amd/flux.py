@@ -43,14 +43,18 @@ def integrate_planes(ctx, grids, bfields, spin="total", precision="fp64"):
                            precision=precision)
 
 
-def report(net, pos, neg):
-    """the reference's block (integral.f90:167-183)"""
-    lines = ["*" * 60,
-             "   Induced current (au)    :%13.6f" % net,
-             "      Positive contribution:%13.6f  (%11.6f )" % (pos, au2si(pos)),
-             "      Negative contribution:%13.6f  (%11.6f )" % (neg, au2si(neg)),
-             "",
-             "   Induced current (nA/T)  :%13.6f" % au2si(net),
-             "      (conversion factor)  :%13.6f" % au2si(1.0),
-             "*" * 60]
-    return "\n".join(lines)
+def report(net, pos, neg, modulus=False):
+    """the reference's block for one flux (integral.f90:167-183; the
+    modulus' labels from integral.f90:306-322), as its msg_out prints it
+    (one leading blank, trailing blanks trimmed), with the empty records
+    before and after it; a list of lines"""
+    au = "Induced mod current (au)   :" if modulus else "   Induced current (au)    :"
+    si = "Induced mod current (nA/T) :" if modulus else "   Induced current (nA/T)  :"
+    return ["", " " + "*" * 60,
+            " " + au + "%13.6f" % net,
+            " " + "      Positive contribution:%13.6f  (%11.6f )" % (pos, au2si(pos)),
+            " " + "      Negative contribution:%13.6f  (%11.6f )" % (neg, au2si(neg)),
+            "",
+            " " + si + "%13.6f" % au2si(net),
+            " " + "      (conversion factor)  :%13.6f" % au2si(1.0),
+            " " + "*" * 60, ""]

@@ -10,6 +10,8 @@ Follows src/fgimic/grid.f90 and src/fgimic/magnet.f90:
   * point order i fastest, then j, then k (grid.f90:478-495)
   * get_magnet/check_field (magnet.f90:11-79): B flipped to point against
     the grid normal unless it is the 'X' (orthogonal) axis
+  * the lines the reference prints while it builds the grid (Grid.log) and
+    chooses the field (get_magnet's log), in its formats
 All lengths in bohr.
 """
 import math
@@ -46,6 +48,7 @@ class Grid:
         self.radius = 1.0e10
         self.gauss = False
         self.xdata = None
+        self.log = []                  # stdout lines of the set-up
 
     # -- geometry ------------------------------------------------------------
     def points(self):
@@ -87,9 +90,14 @@ class Grid:
         return int(np.prod(self.npts))
 
 
+def _fmt(spec, v):
+    return "".join(spec % x for x in v)
+
+
 def _rotate(g, angle, reference):
     """rotate, grid.f90:697-769 (right-hand rule, z then y then x)"""
     rad = np.asarray(angle, dtype=np.float64) / 180.0 * PII
+    g.log.append(" INFO: Rotation is: " + _fmt("%9.5f", rad))
     rot = np.zeros((3, 3))
     x = rad[2]
     rot[0, 0] = math.cos(x)
@@ -134,6 +142,7 @@ def _setup_even(g):
 
 
 def _setup_gauss(g, kw, quadr):
+    g.log.append(" INFO: Integration grid selected.")
     order = int(kw.get("gauss_order", 7))
     if "grid_points" in kw:
         npts = [int(v) for v in kw["grid_points"]]
@@ -141,12 +150,16 @@ def _setup_gauss(g, kw, quadr):
         spc = kw["spacing"]
         npts = [0 if (abs(spc[i]) < 1e-10 or spc[i] < 0) else int(round(g.l[i] / spc[i]))
                 for i in range(3)]
+    fixed = False
     for i in range(3):
         if not npts[i] > 1:
             npts[i] = 0
         rem = npts[i] % order
         if rem:
             npts[i] = npts[i] - rem + order
+            fixed = True
+    if fixed:
+        g.log.append(" INFO: Adjusted number of grid points for quadrature: " + _fmt("%5d", npts))
     g.gauss = True
     g.npts = np.zeros(3, dtype=int)
     for i in range(3):
@@ -174,6 +187,7 @@ def new_grid(kw, coords=None, magnet=None):
         g.gtype = "file"
         g.xdata = np.loadtxt(kw["file"]).reshape(-1, 3)
         g.npts = np.array([len(g.xdata), 1, 1])
+        g.log += ["   Total number of grid points  :%10d" % g.size, ""]   # grid.f90:572-575
         return g
     if g.mode in ("std", "base"):
         g.origin = np.asarray(kw["origin"], dtype=np.float64)
@@ -224,8 +238,15 @@ def new_grid(kw, coords=None, magnet=None):
         if magnet is not None:
             g.ortho = _norm(magnet)
         g.bond_center = oo
+        # grid.f90:258-274, before any rotation
+        g.log += ["", " Integration grid data", " " + "-" * 48]
+        for name, v in (("center ", oo), ("origin ", g.origin), ("basv1  ", v1), ("basv2  ", v2),
+                        ("basv3  ", v3), ("lenghts", g.l), ("magnet ", g.ortho)):
+            g.log.append(" " + name + _fmt("%12.6f", v))
+        g.log.append("")
     else:
         raise ValueError(f"Unknown grid type: {g.mode}")
+    g.log.append(" Grid mode = " + g.mode)
     # normalise + ortho_coordsys (grid.f90:286-300, 398-423)
     for i in range(3):
         n = math.sqrt(float(np.sum(g.basv[:, i] ** 2)))
@@ -245,7 +266,12 @@ def new_grid(kw, coords=None, magnet=None):
             if "height" in kw:
                 down_length, out_length = float(kw["height"][1]), float(kw["width"][1])
             else:
-                out_length, down_length = float(kw["out"]), float(kw["down"])
+                # Grid.out and Grid.down are not keywords of the input
+                # template (gimic.in:90-112): the reference's getkw leaves
+                # its locals unassigned (getkw.f90:598-623).  0 puts the
+                # rotation reference at the grid origin, which a zero
+                # rotation leaves exactly in place (test/benzene/vectors)
+                out_length, down_length = float(kw.get("out", 0.0)), float(kw.get("down", 0.0))
             ref = g.origin + out_length * g.basv[:, 1] + down_length * g.basv[:, 0]
         _rotate(g, kw["rotation"], ref)
     g.gtype = kw.get("type", "even")
@@ -255,11 +281,13 @@ def new_grid(kw, coords=None, magnet=None):
         _setup_gauss(g, kw, g.gtype)
     else:
         raise ValueError(f"Unknown grid type: {g.gtype}")
+    g.log += ["   Number of grid points <v1,v2>:" + _fmt("%5d", g.npts),
+              "   Total number of grid points  :%10d" % g.size, ""]      # grid.f90:131-137
     return g
 
 
-def get_magnet(g, magnet_axis="", magnet=None):
-    """magnet.f90:11-79"""
+def get_magnet(g, magnet_axis="", magnet=None, log=None):
+    """magnet.f90:11-79; `log` (a list) receives check_field's lines"""
     axis = (magnet_axis or "").strip()
     if axis in ("T",):
         axis = "X"
@@ -284,7 +312,16 @@ def get_magnet(g, magnet_axis="", magnet=None):
     if np.all(np.abs(mag) <= 1e-12):
         raise ValueError("Magnetic field is zero, not wasting more CPU.")
     mag = np.asarray(mag, dtype=np.float64)
-    if not ortho and g.mode != "file":
-        if float(g.basv[:, 2] @ mag) > 0.0:
+    if not ortho:
+        # a file grid has no basis vectors (grid.f90:548): x = 0, no flip
+        x = float(g.basv[:, 2] @ mag)
+        if x > 0.0:
             mag = -mag
+        if log is not None:
+            # check_field, magnet.f90:64-86 (its warning fires only for |x| > 1)
+            if x > 0.0:
+                log.append(" INFO: Left handed coordinate system, reversing magnetic field")
+            if abs(x) - 1.0 > 1e-12 and abs(x) > 1e-12:
+                log.append(" WARNING: Magnetic field not orthogonal to grid")
+            log += ["    Magnetic field <x,y,z> =" + _fmt("%10.5f", mag), ""]
     return mag

@@ -14,6 +14,7 @@
                   6f11.7)
   grid.xyz        atoms and grid corners in angstrom (plot_grid_xyz,
                   grid.f90:588-670)
+  mol.xyz         the atoms in angstrom (write_xyz, basis.f90:487-511)
 
 The text layouts follow what gfortran writes for the reference's format
 statements: Ew.d as 0.dddddd mantissas, list-directed records opening with a
@@ -27,8 +28,12 @@ import os
 
 import numpy as np
 
-AU2A = 0.52917726  # angstrom per bohr (globals.f90:51)
-DP33 = 0.3333333   # globals.f90:62
+# angstrom per bohr (globals.f90:51).  The reference's literal 0.52917726 has
+# no D exponent, so it is a default (single precision) real that the
+# REAL(DP) parameter widens: 0.52917724847793579..., which is what its
+# xyz files and jmod.txt coordinates carry in their last printed digits
+AU2A = float(np.float32(0.52917726))
+DP33 = 0.3333333   # globals.f90:62 (0.3333333D0, a double)
 
 
 # ---- Fortran edit descriptors ------------------------------------------------
@@ -288,13 +293,30 @@ def acid_values(jt):
     return DP33 * (d1 + d2 + d3) + 0.5 * (o1 + o2 + o3)
 
 
+def _xyz_line(symbol, r_bohr):
+    """'(a, 3f16.10)' of a symbol and a position converted to angstrom"""
+    return symbol + "".join(fortran_f(x * AU2A, 16, 10) for x in r_bohr) + "\n"
+
+
+def write_mol_xyz(fname, symbols, coords):
+    """write_xyz (basis.f90:487-511), written when the basis is read
+    (basis.f90:84): the atom count as a list-directed record, an empty
+    record, then each atom's two-character symbol and position in angstrom"""
+    coords = np.asarray(coords, dtype=np.float64).reshape(-1, 3)
+    with open(fname, "w") as f:
+        f.write(list_record(len(coords)))
+        f.write("\n")
+        for s, r in zip(symbols, coords):
+            f.write(_xyz_line(s[:2].ljust(2), r))
+
+
 def write_grid_xyz(fname, g, symbols, coords):
     """plot_grid_xyz (grid.f90:588-670): atoms, the grid's corners as 'X'
     and a 'Be' marker two bohr along the grid normal (std/base) or the
     orthogonal axis (bond), in angstrom"""
     coords = np.asarray(coords, dtype=np.float64).reshape(-1, 3)
     p1, p2, p3 = (int(v) for v in g.npts)
-    line = lambda s, r: s + "".join(fortran_f(x * AU2A, 16, 10) for x in r) + "\n"
+    line = _xyz_line
     nat = len(coords)
     with open(fname, "w") as f:
         if p3 > 1:

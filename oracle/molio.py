@@ -109,7 +109,7 @@ def read_mol(path):
             hdr += rec.line().replace(",", " ").split()
         nctrps = [int(x) for x in hdr[3:3 + nsh]]
         sline = rec.line()
-        b.symbols.append(sline[0:2].strip())
+        b.symbols.append(sline[0:2].ljust(2))   # the two columns as read (intgrl.f90:112)
         b.charges.append(charge)
         b.coords.append([_num(x) for x in sline[4:].split()[:3]])
         for ish in range(nsh):

@@ -87,3 +87,42 @@ def runtest_mismatches(out_text, ref_text, rel_tolerance):
         if fy == 0.0 or abs(fx - fy) > rel_tolerance * abs(fy):
             bad.append((x, y))
     return bad
+
+
+def stdout_setup_lines(text):
+    """the lines the reference prints between reading the density and naming
+    the shell type (gimic.F90:159-171): the grid's set-up, the grid plot
+    note and the field check"""
+    lines = text.splitlines()
+    end = next(i for i, l in enumerate(lines) if "-shell calculation" in l)
+    # the first line new_grid prints: the empty record before a bond grid's
+    # data block, the mode, or a file grid's point count
+    start = min(i - (l == " Integration grid data") for i, l in enumerate(lines[:end])
+                if l == " Integration grid data" or l.startswith(" Grid mode = ")
+                or l.startswith("   Total number of grid points"))
+    return lines[start:end]
+
+
+def stdout_integral_lines(text):
+    """the printout of run_integral (gimic.F90:222-261): from 'Integrating
+    current density' up to the deallocation notes (or the end)"""
+    lines = text.splitlines()
+    start = next(i for i, l in enumerate(lines) if l == " Integrating current density")
+    end = next((i for i, l in enumerate(lines) if "Deallocated grid data" in l), len(lines))
+    return lines[start:end]
+
+
+def printout_mismatches(ours, ref, atol):
+    """line-by-line comparison of two printouts: the text around the numbers
+    exactly, every number to `atol`.  Returns the differing line pairs."""
+    bad = []
+    if len(ours) != len(ref):
+        return [(f"{len(ours)} lines", f"{len(ref)} lines")]
+    for a, r in zip(ours, ref):
+        if a == r:
+            continue
+        na, nr = _NUM.findall(a), _NUM.findall(r)
+        if _NUM.sub("#", a) != _NUM.sub("#", r) or len(na) != len(nr) or any(
+                abs(float(x) - float(y)) > atol for x, y in zip(na, nr)):
+            bad.append((a, r))
+    return bad

@@ -11,13 +11,21 @@ the reference tree; the GPU box only reads the files written here).
                   bytes): open-shell/3d/reference/{jvec,jmod}[tag].vti,
                   c4h4/read-grid/reference/jvec.vtu, and the TetGen cells
                   c4h4/read-grid/grid.1.ele that run reads
+  benzene.npz     what the benzene tests and example hold that needs no
+                  density (their XDENS is not in the tree): test/benzene/MOL,
+                  each test's input, reference stdout and VTI headers (up to
+                  the DataArray line), int-cdens/reference/jmod.txt, every
+                  29th point of the 3d and 3d-keyword-magnet jvec.vti /
+                  jmod.vti values and of the 2d pair's jvec.vti values (as
+                  the files print them), and examples/benzene/{int,3D}
+                  inputs with the mol.xyz and grid.xyz they produced
   ring12_ref.npz  flang-built reference (oracle/_ref) tensors, all nine
                   components, for the synthetic 12-atom ring at 256 seeded
                   points, the cartesian-equivalent density it was fed, and
                   composite Gauss/Lobatto rules from the reference's
                   setup_gauss_data
 
-Usage: python tests/golden/make_golden.py [/root/reference] [--writers-only]
+Usage: python tests/golden/make_golden.py [/root/reference] [--writers-only|--benzene-only]
 """
 import lzma
 import os
@@ -85,6 +93,55 @@ def writer_files(ref):
     np.savez(os.path.join(HERE, "writers.npz"), **out)
 
 
+BENZENE_TESTS = ["2d", "2d-keyword-magnet", "3d", "3d-keyword-magnet", "diamag-off", "giao-test",
+                 "int-cdens", "int-grid-bond-even", "integration-gauss", "integration-lobatto",
+                 "keyword-magnet", "keyword-radius", "keyword-rotation", "keyword-rotation_origin",
+                 "keyword-spacing", "paramag-off", "skip-jmod-integration", "vectors"]
+SAMPLE_STRIDE = 29
+
+
+def data_tokens(text, name):
+    """the printed values of DataArray `name`, as text tokens"""
+    i = text.index(f'Name="{name}"')
+    i = text.index(">", i) + 1
+    return text[i:text.index("</DataArray>", i)].split()
+
+
+def benzene_files(ref):
+    """benzene.npz, named in the module docstring"""
+    t = os.path.join(ref, "test", "benzene")
+    out = {"mol": np.array(open(os.path.join(t, "MOL")).read())}
+    for name in BENZENE_TESTS:
+        d = os.path.join(t, name)
+        inp = "vectors.inp" if name == "vectors" else "gimic.inp"
+        out[f"{name}/input"] = np.array(open(os.path.join(d, inp)).read())
+        rd = os.path.join(d, "reference")
+        for f in sorted(os.listdir(rd)):
+            text = open(os.path.join(rd, f)).read()
+            if f == "stdout":
+                out[f"{name}/stdout"] = np.array(text)
+            elif f.endswith(".vti"):
+                k = text.index("<DataArray")
+                out[f"{name}/{f}:head"] = np.array(text[:text.index("\n", k) + 1])
+    out["int-cdens/jmod.txt"] = np.array(open(os.path.join(t, "int-cdens", "reference",
+                                                         "jmod.txt")).read())
+    for name, files in (("3d", ("jvec", "jmod")), ("3d-keyword-magnet", ("jvec", "jmod")),
+                        ("2d", ("jvec",)), ("2d-keyword-magnet", ("jvec",))):
+        for f in files:
+            text = open(os.path.join(t, name, "reference", f + ".vti")).read()
+            tok = data_tokens(text, "vectors" if f == "jvec" else "scalars")
+            ncomp = 3 if f == "jvec" else 1
+            pts = range(0, len(tok) // ncomp, SAMPLE_STRIDE)
+            out[f"{name}/{f}.vti:sample"] = np.array(
+                " ".join(" ".join(tok[p * ncomp:(p + 1) * ncomp]) for p in pts))
+    out["sample_stride"] = np.array(SAMPLE_STRIDE)
+    ex = os.path.join(ref, "examples", "benzene")
+    for d in ("int", "3D"):
+        for f in ("gimic.inp", "mol.xyz", "grid.xyz"):
+            out[f"examples/{d}/{f}"] = np.array(open(os.path.join(ex, d, f)).read())
+    np.savez_compressed(os.path.join(HERE, "benzene.npz"), **out)
+
+
 def main(ref):
     t = os.path.join(ref, "test")
     # ---- c4h4 ----
@@ -112,6 +169,7 @@ def main(ref):
     )
     np.savez_compressed(os.path.join(HERE, "openshell.npz"), **out)
     writer_files(ref)
+    benzene_files(ref)
     # ---- synthetic ring through the flang-built reference ----
     from gimic_amd import synth
     from oracle import molio, refrun, sph
@@ -154,5 +212,7 @@ if __name__ == "__main__":
     ref = args[0] if args else "/root/reference"
     if "--writers-only" in sys.argv:
         writer_files(ref)
+    elif "--benzene-only" in sys.argv:
+        benzene_files(ref)
     else:
         main(ref)

@@ -7,7 +7,7 @@ import re
 import numpy as np
 import pytest
 
-from conftest import golden, write_fixture
+from conftest import golden, stdout_setup_lines, write_fixture
 from gimic_amd import grid as gridmod, inp
 from oracle import molio, oracle
 
@@ -75,6 +75,37 @@ def test_bond_grid_geometry_matches_reference_printout(tmp_path, name, uhf):
     m = re.search(r"Adjusted number of grid points for quadrature: +(\d+) +(\d+) +(\d+)",
                   str(d["integration_stdout"]))
     assert [int(m.group(1)), int(m.group(2))] == list(g.npts[:2])
+
+
+@pytest.mark.parametrize("name,uhf", [("c4h4.npz", False), ("openshell.npz", True)])
+def test_setup_lines_match_reference_printout(tmp_path, name, uhf):
+    """new_grid's and check_field's lines as the reference prints them
+    (c4h4: a zero rotation; both: the 'X' axis, no field check)"""
+    d, b, dens, kw, g, B = fixture_setup(tmp_path, name, uhf)
+    mlog = []
+    gridmod.get_magnet(g, kw.get("magnet_axis", ""), kw.get("magnet"), log=mlog)
+    want = stdout_setup_lines(str(d["integration_stdout"]))
+    assert g.log + [" *** Grid plot in grid.xyz"] + mlog == want
+
+
+@pytest.mark.parametrize("name,uhf", [("c4h4.npz", False), ("openshell.npz", True)])
+def test_integral_printout_layout(tmp_path, name, uhf):
+    """driver.integral_report rebuilt from the printed values: the open-shell
+    run's four spin cases with |J|, byte for byte around the numbers"""
+    from conftest import printout_mismatches, stdout_integral_lines
+    from gimic_amd import driver
+    d, b, dens, kw, g, B = fixture_setup(tmp_path, name, uhf)
+    text = str(d["integration_stdout"])
+    cur = stdout_blocks(text, "Induced current (au)")
+    mod = stdout_blocks(text, "Induced mod current (au)")
+    res = {spin: dict(current=cur[k], modulus=mod[k], acid=None)
+           for k, spin in enumerate(driver.spin_cases(uhf))}
+    # c4h4's printout is from a version that integrated |J| unasked
+    jmod = bool(kw["Essential"]["jmod"]) or name == "c4h4.npz"
+    mlog = []
+    gridmod.get_magnet(g, kw.get("magnet_axis", ""), kw.get("magnet"), log=mlog)
+    ours = driver.integral_report(res, uhf, jmod, bool(kw["Essential"]["acid"]), g.radius, mlog)
+    assert printout_mismatches(ours, stdout_integral_lines(text), 1.5e-5) == []
 
 
 @pytest.mark.parametrize("name,uhf", [("c4h4.npz", False), ("openshell.npz", True)])

@@ -8,7 +8,9 @@ directory) with the reference's own test inputs:
     with acid=on also acid.vti, against the same file written from the
     oracle's tensors;
   * test/c4h4/read-grid (file grid, B = (0, 0, -1)): jvec.vtu at rel 1e-8;
-  * grid.xyz in both runs.
+  * grid.xyz in both runs, mol.xyz against the oracle's reading of MOL;
+  * test/{c4h4,open-shell}/integration: the set-up lines and flux blocks of
+    the printout.
 """
 
 import numpy as np
@@ -58,6 +60,11 @@ def test_driver_writes_openshell_3d_files(tmp_path, monkeypatch):
     assert not (tmp_path / "acid.vti").exists()
     lines = (tmp_path / "grid.xyz").read_text().splitlines()
     assert int(lines[0]) == len(lines) - 2
+    # mol.xyz from the library's reading of MOL (gimic_ctx_atom_symbols,
+    # gimic_ctx_atoms) equals the one written from the oracle's reading
+    b = molio.read_mol(str(tmp_path / "MOL"))
+    writers.write_mol_xyz(str(tmp_path / "mol_oracle.xyz"), b.symbols, b.coords)
+    assert (tmp_path / "mol.xyz").read_text() == (tmp_path / "mol_oracle.xyz").read_text()
 
 
 def test_driver_acid_vti_matches_oracle_written_file(tmp_path, monkeypatch):
@@ -93,6 +100,34 @@ def test_driver_writes_readgrid_vtu(tmp_path, monkeypatch):
     ref = writer_fixture("c4h4/read-grid/reference/jvec.vtu")
     assert runtest_mismatches(out, ref, 1e-8) == []
     assert (tmp_path / "grid.xyz").exists()
+
+
+@pytest.mark.parametrize("name", ["c4h4.npz", "openshell.npz"])
+def test_driver_integral_run_prints_reference_lines(tmp_path, monkeypatch, capsys, name):
+    """test/{c4h4,open-shell}/integration through driver.main: the set-up
+    lines byte for byte; open-shell's whole integral printout (jmod=on, four
+    spin cases) around the numbers byte for byte, the numbers to the printed
+    digits (one unit of the sixth decimal in au, 28.2 times that in nA/T);
+    c4h4's input leaves jmod off (its printout is from a version that
+    integrated |J| unasked): its current block"""
+    from conftest import printout_mismatches, stdout_integral_lines, stdout_setup_lines
+    from test_cpu_grid_flux import stdout_blocks
+    write_fixture(str(tmp_path), name)
+    d = golden(name)
+    print(f"case driver {name} integration", flush=True)
+    _run_driver(tmp_path, monkeypatch, str(d["integration_inp"]))
+    out = capsys.readouterr().out
+    ref = str(d["integration_stdout"])
+    assert stdout_setup_lines(out) == stdout_setup_lines(ref)
+    if name == "openshell.npz":
+        assert printout_mismatches(stdout_integral_lines(out), stdout_integral_lines(ref),
+                                   3e-5) == []
+    else:
+        assert " Jmod integration skipped." in out.splitlines()
+        ours = stdout_blocks(out, "Induced current (au)")
+        want = stdout_blocks(ref, "Induced current (au)")
+        assert len(ours) == len(want) == 1
+        assert np.allclose(ours, want, rtol=0, atol=1.01e-6), (ours, want)
 
 
 # READGRID_INP as the reference's frontend pipes it into gimic.bin
