@@ -73,11 +73,12 @@ def time_steps(ctx, d_pts, out, precision, steps, warmup):
 
 
 def kernel_roofline(ctx, d_pts, out, precision, nbf):
-    """Two extra passes: one that counts the screening block lists of every
-    row tile (a tile of R rows with n listed 32-function blocks multiplies
-    R x 32n by 32n x 7*32n: 14 R (32 n)^2 flops), and one with event timing
-    around each kernel launch.  achieved = listed-block flops per launch /
-    average launch time; the dense product would need 14 nbf^2 per point."""
+    """Two extra passes: one that counts the screening lists of every row
+    tile (a tile of R rows with k live 8-function groups multiplies
+    R x 8k by 8k x 7*8k: 14 R (8 k)^2 flops; both kernels walk the live
+    groups along N and K), and one with event timing around each kernel
+    launch.  achieved = listed flops per launch / average launch time; the
+    dense product would need 14 nbf^2 per point."""
     import torch
     # these passes measure the mixed contraction kernel alone: the FP64 pass
     # over points next to a nucleus (counted in the step time) is left out
@@ -98,11 +99,10 @@ def kernel_roofline(ctx, d_pts, out, precision, nbf):
     npts = d_pts.shape[0]
     dense = 14.0 * nbf * nbf * npts
     if ts["tiles"] > 0:
-        # both the pre-split mixed kernels and the FP64 kernel step K through
-        # the live 8-function groups of the listed blocks
-        kn = ts["blocks_kn"]
-        flop = 14.0 * ts["tile_rows"] * 1024.0 * kn
-        block_pairs = kn / (ts["tiles"] * ts["nblk"] ** 2)
+        # both kernels walk the live 8-function groups along N and K
+        pairs = ts["groups_sq"] * 64.0
+        flop = 14.0 * ts["tile_rows"] * pairs
+        block_pairs = pairs / (ts["tiles"] * (32.0 * ts["nblk"]) ** 2)
     else:  # a dense kernel
         flop, block_pairs = dense, 1.0
     avg_ms = ms / launches

@@ -308,7 +308,7 @@ struct gimic_ctx {
     int mixed_split = 0;  // N-split count of the pre-split kernel (0: by workgroup count)
     // block-list statistics (gimic_ctx_set_tile_stats): tiles, sum n, sum n^2, rows per tile
     bool tile_stats = false;
-    double tile_acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double tile_acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     float tune_ms[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     std::vector<std::pair<int, float>> tune_log;  // (variant * 100 + split, ms)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
@@ -529,12 +529,14 @@ gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec)
     } else {
         std::vector<float> b32(B.begin(), B.end());
         // the FP32 values as their first kBParts BF16 parts (three parts:
-        // x = h + m + l exactly), one 8-function group per (set, N-tile,
-        // group kg), [part][column][8], column = matrix * 32 + function
-        // within the N-tile; the mixed kernels gather four groups per K-step
-        // into their LDS buffers
-        const size_t nks = ld / 32, ntl = nc / 32, grp = kMx2GroupElems;
-        std::vector<uint16_t> bs(spins.size() * ntl * nks * kMx2StepElems + 1024, 0);
+        // x = h + m + l exactly), one pair block per (set, 8-function K-group
+        // gk, 8-function N-group gn), [part][matrix][n][k]; rows of K-groups
+        // of ngn + 1 blocks (the last zero) and a zero row last: the mixed
+        // kernels gather the 16 blocks of an N-tile's K-step into their LDS
+        // buffers
+        const size_t nkg = ld / 8, ngn = nc / 8, pe = kMx2PairElems;
+        const size_t setel = mx2_set_elems((int)ld, (int)nc);
+        std::vector<uint16_t> bs(spins.size() * setel, 0);
         auto bf = [](float x) {
             uint32_t u;
             std::memcpy(&u, &x, 4);
@@ -547,22 +549,19 @@ gimic_ctx::Mats &get_mats(gimic_ctx *c, const std::vector<int> &spins, int prec)
             std::memcpy(&x, &u, 4);
             return x;
         };
-        const size_t part = (size_t)kMx2Cols * 8;
         for (size_t si = 0; si < spins.size(); si++)
-            for (size_t nt = 0; nt < ntl; nt++)
-                for (int col = 0; col < kMx2Cols; col++) {
-                    const size_t mat = col / 32, nu = nt * 32 + col % 32;
+            for (size_t mat = 0; mat < (size_t)kNMat; mat++)
+                for (size_t nu = 0; nu < nc; nu++) {
                     const float *src = &b32[si * setsz + mat * matsz + nu * ld];
-                    for (size_t kg = 0; kg < 4 * nks; kg++) {
-                        uint16_t *dst = &bs[((si * ntl + nt) * 4 * nks + kg) * grp];
-                        for (int e = 0; e < 8; e++) {
-                            float x = src[kg * 8 + e];
-                            const size_t o = (size_t)col * 8 + e;
-                            for (int p = 0; p < kBParts; p++) {
-                                const uint16_t h = bf(x);
-                                dst[p * part + o] = h;
-                                x -= fl(h);
-                            }
+                    const size_t gn = nu / 8, nn = nu % 8;
+                    for (size_t mu = 0; mu < ld; mu++) {
+                        const size_t gk = mu / 8, kk = mu % 8;
+                        uint16_t *dst = &bs[si * setel + (gk * (ngn + 1) + gn) * pe];
+                        float x = src[mu];
+                        for (int p = 0; p < kBParts; p++) {
+                            const uint16_t h = bf(x);
+                            dst[((p * kNMat + mat) * 8 + nn) * 8 + kk] = h;
+                            x -= fl(h);
                         }
                     }
                 }
@@ -872,6 +871,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
                     c->tile_acc[1] += k;
                     c->tile_acc[2] += (double)k * k;
                     c->tile_acc[4] += (double)k * cnt8[t] / 4.0;
+                    c->tile_acc[5] += (double)cnt8[t] * cnt8[t];
                 }
                 c->tile_acc[3] = tile_rows;
             }
@@ -1689,6 +1689,7 @@ int gimic_ctx_tile_stats(const gimic_ctx *c, double *out) {
         for (int k = 0; k < 4; k++) out[k] = c->tile_acc[k];
         out[4] = c->lda / 32;
         out[5] = c->tile_acc[4];
+        out[6] = c->tile_acc[5];
     });
 }
 

@@ -19,9 +19,9 @@
 // (32 per wavefront: two 16-row MFMA blocks) and walks a range of 16-column
 // N-tiles; per N-tile each wavefront keeps 2 x 7 v_mfma_f64_16x16x4 blocks.
 // Only what screening leaves non-zero over the row tile (tiles.hip) is
-// visited: along N the listed 32-function blocks' 16-column halves, split
-// evenly over the workgroups that share the row tile; along K the live
-// 8-function groups of the listed blocks, two per 16-wide K-step.
+// visited: the live 8-function groups of its listed 32-function blocks,
+// two per 16-column N-tile (split evenly over the workgroups that share
+// the row tile) and two per 16-wide K-step.
 // The K index is remapped inside every K-step so each lane loads 32
 // contiguous bytes of its A row and of each B column (lane group g holds
 // elements 4(g&1)..+3 of group g/2; sub-step s feeds the s-th of them to
@@ -36,8 +36,6 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs a) {
     const int g = lane >> 4, c = lane & 15;
     const int set = blockIdx.z, split = blockIdx.y;
     const size_t row0 = (size_t)blockIdx.x * kWgRows;
-    const int *L = a.blist + (size_t)blockIdx.x * a.nblk;
-    const int nk = 2 * a.bcount[blockIdx.x];  // 16-wide K-steps (and N-tiles) listed
     const int lda = a.lda;
     const size_t rows = (size_t)a.rows;
     const size_t matsz = (size_t)a.ncol_pad * lda;
@@ -48,23 +46,24 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs a) {
     const double *A0 = a.a64 + (row0 + wave * 32 + c) * lda;
     const double *A1 = A0 + 16 * (size_t)lda;
     const double *Bset = a.b64 + (size_t)set * kNMat * matsz;
-    const int e0 = (int)((long long)split * nk / a.nsplit);
-    const int e1 = (int)((long long)(split + 1) * nk / a.nsplit);
-    // first element of listed N-tile e (the listed blocks' 16-column halves)
-    auto kof = [&](int kk) { return L[kk >> 1] * 32 + (kk & 1) * 16; };
-    // K walks the live 8-function groups of the listed blocks, two per
-    // 16-wide K-step (tiles.hip): lane group g takes group 2 kk + g / 2,
-    // elements 4 (g & 1) .. + 3.  A padding entry (-1) reads A from the
-    // zero buffer (B from any valid place: the product is zero).
+    // the live 8-function groups of the listed blocks (tiles.hip), two per
+    // 16-wide K-step: lane group g takes group 2 kk + g / 2, elements
+    // 4 (g & 1) .. + 3.  A padding entry (-1) reads A from the zero buffer
+    // (B from any valid place: the product is zero).  Two per N-tile too:
+    // lane c owns function c & 7 of group 2 e + c / 8 (a padding entry: a
+    // column whose products the epilogue drops).
     const int *K8 = a.k8list + (size_t)blockIdx.x * a.nblk * 4;
     const int nks = (a.k8count[blockIdx.x] + 1) >> 1;
+    const int e0 = (int)((long long)split * nks / a.nsplit);
+    const int e1 = (int)((long long)(split + 1) * nks / a.nsplit);
     const double *zero = (const double *)a.bzero;
     auto kent = [&](int kk) { return K8[2 * kk + (g >> 1)]; };
     auto koff = [&](int v) { return v < 0 ? 0 : (v >> 12) * 8 + (g & 1) * 4; };
 
     for (int e = e0; e < e1; e++) {
-        const int nt = kof(e) / kNT;
-        const int col = nt * kNT + c;
+        const int xn = K8[2 * e + (c >> 3)];
+        const bool live = xn >= 0;
+        const int col = live ? (xn >> 12) * 8 + (c & 7) : 0;
         const double *Bc = Bset + (size_t)col * lda;
         v4d acc[2][kNMat];
 #pragma unroll
@@ -113,9 +112,10 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs a) {
                 const bool valid = p < (size_t)a.npts;
                 const double rp[3] = {valid ? a.pts[3 * p] : 0.0, valid ? a.pts[3 * p + 1] : 0.0,
                                       valid ? a.pts[3 * p + 2] : 0.0};
-                const double ph = a.a64[p * lda + col];
-                const double G[3] = {a.g64[p * lda + col], a.g64[gplane + p * lda + col],
-                                     a.g64[2 * gplane + p * lda + col]};
+                const double ph = live ? a.a64[p * lda + col] : 0.0;
+                const double G[3] = {live ? a.g64[p * lda + col] : 0.0,
+                                     live ? a.g64[gplane + p * lda + col] : 0.0,
+                                     live ? a.g64[2 * gplane + p * lda + col] : 0.0};
                 const double y0 = acc[rb][0][r];
                 double q[kNQ];
                 q[0] = y0 * ph;

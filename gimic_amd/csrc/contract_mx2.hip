@@ -9,19 +9,22 @@
 //   * the basis kernel writes the left operand as [row][k/8][kAParts][8]
 //     (16 kAParts contiguous bytes per lane and K-step), and
 //   * the right operand (constant per density) is split once on the host into
-//     groups of kBParts x 3.5 KiB per (density set, N-tile, 8-function
-//     group), [part][column][8]; a K-step's LDS buffer holds four of them.
-// The K-loop therefore has no conversion work: each K-step streams the four
-// groups of a later step into a free LDS buffer with gfx950's direct
-// global-to-LDS loads (global_load_lds_dwordx4: no VGPRs, no ds_write), loads
+//     pair blocks of kBParts x 896 bytes per (density set, 8-function
+//     K-group, 8-function N-group), [part][matrix][n][8 k]; a K-step's LDS
+//     buffer gathers the 16 pair blocks of its four K-groups and the N-tile's
+//     four N-groups.
+// The K-loop therefore has no conversion work: each K-step streams the 16
+// pair blocks of a later step into a free LDS buffer with gfx950's direct
+// global-to-LDS loads (global_load_lds_dwordx4: no VGPRs, no ds_write; each
+// lane names its own 16-byte source, so the gather costs nothing), loads
 // the next A fragments, and runs KRB x 14 x split_products() MFMAs per
 // wavefront against B fragments that are read from LDS one column block
 // ahead.  NBUF = 2 streams block ks+1 during K-step ks; NBUF = 3 streams
 // block ks+2, so a block has two K-steps to arrive from the Infinity Cache.
 //
 // Tile: 16 * KRB * WAVES point rows per workgroup (WAVES wavefronts x 16 KRB
-// rows), 224 columns per N-tile (7 matrices x 32 functions), KRB x 14 MFMA
-// blocks per wavefront.  KRB = 4 reads every B fragment from LDS once for 4
+// rows), 224 columns per N-tile (7 matrices x four 8-function groups), KRB x
+// 14 MFMA blocks per wavefront.  KRB = 4 reads every B fragment from LDS once for 4
 // row blocks (half the LDS traffic per MFMA of KRB = 2) and doubles the work
 // per K-step that hides the next step's loads; it needs 224 accumulator
 // registers.  WAVES = 8 (256 rows, two wavefronts per SIMD) lets one
@@ -38,16 +41,17 @@
 // with 256-row tiles).
 
 //
-// Screening.  A workgroup visits only the 32-function blocks listed for its
-// row tile (tiles.hip): the listed N-blocks are split evenly over the nsplit
-// workgroups of the row tile, and the K-loop walks the live 8-function
-// groups of the listed blocks, four per K-step (an MFMA's K of 32): groups
-// in which screening zeroes every function of the tile are skipped as
-// well, which removes a fifth of the K-steps on the 60-atom cage.  Each
-// lane group of a wavefront reads its own group's A fragment (the group's
-// place in the compressed row), and the buffer copy gathers the four
-// groups.  The LDS buffers rotate over K-steps, so the pipeline is the same
-// as for a dense walk.
+// Screening.  A workgroup visits only the live 8-function groups of its row
+// tile (tiles.hip: the groups of the listed 32-function blocks in which
+// screening leaves some function of the tile non-zero), on both sides of
+// the product: an N-tile is four of them (their 32 functions x 7 matrices),
+// the N-tiles are split evenly over the nsplit workgroups of the row tile,
+// and the K-loop walks the same groups, four per K-step (an MFMA's K of
+// 32).  On the 60-atom cage that is a fifth fewer (N-tile, K-step) pairs
+// than N-tiles of whole 32-function blocks.  Each lane group of a wavefront
+// reads its own group's A fragment (the group's place in the compressed
+// row), and the buffer copy gathers the 16 pair blocks.  The LDS buffers
+// rotate over K-steps, so the pipeline is the same as for a dense walk.
 //
 // Workgroup order.  Every workgroup re-reads its rows of A once per N-tile,
 // and every row tile streams the B blocks of its listed (N, K) pairs.  The
@@ -74,9 +78,15 @@ constexpr int kCB = kCols / 16;             // 14 column blocks
 constexpr int kBlk = kMx2BlockElems;        // BF16 per block / LDS buffer
 constexpr size_t kBufBytes = (size_t)kBlk * 2;
 // A K-step's LDS buffer is [group slot j][part][column][8]: slot j holds one
-// 8-function group (kBParts x 3584 bytes, contiguous in HBM too), copied in
-// kPPG 1 KiB pieces per slot, the last one partial when the group does not
-// fill it (three parts: 10752 bytes, eleven pieces, the last for lanes 0-31)
+// K-group (kBParts x 3584 bytes), copied in kPPG 1 KiB pieces per slot, the
+// last one partial when the slot does not fill it (three parts: 10752
+// bytes, eleven pieces, the last for lanes 0-31).  Lane l of piece t of a
+// slot fills unit u = 64 t + l of the slot: [part p][column], column =
+// matrix m * 32 + 8 * N-slot + n.  As 224 = 7 * 32, the 32-unit chunk
+// w = u / 32 = 2 t + l / 32 is p * 7 + m, and column % 32 = l % 32, so the
+// lane's N-slot (l / 8) % 4 and function n = l % 8 are the same in every
+// piece: the unit is element (p * 7 + m) * 64 + 8 n of the pair block of the
+// slot's K-group and the lane's N-group, at lane offset nlo + 128 t
 constexpr int kGroupBytes = kMx2GroupElems * 2;
 constexpr int kPPG = (kGroupBytes + 1023) / 1024;           // pieces per group slot
 constexpr int kLastLanes = (kGroupBytes - (kPPG - 1) * 1024) / 16;  // lanes of its last piece
@@ -93,30 +103,43 @@ __device__ __forceinline__ int pieces_of(int wave, int waves) {
 
 typedef __attribute__((address_space(3))) void lds_void;
 
+__device__ __forceinline__ int pick4(int4 v, int i) {
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+}
+
+// the lane's offset, within a K-group's row of pair blocks, of its unit in
+// piece 0 of a slot (fixed for an N-tile): pair block gn of its N-slot's
+// group (ngn, a zero block, for padding), chunk l / 32, function l % 8
+__device__ __forceinline__ int lane_offset(int4 en, int ngn, int lane) {
+    const int x = pick4(en, (lane >> 3) & 3);
+    const int gn = x < 0 ? ngn : x >> 12;
+    return gn * kMx2PairElems + (lane >> 5) * 64 + (lane & 7) * 8;
+}
+
 // piece i of this wavefront's share of a K-step's buffer (64 lanes x 16
-// bytes per instruction, LDS base in M0): piece q = kPPG j + t copies bytes
-// [1024 t, 1024 t + 1024) of group slot j from global group ent[j] >> 12 of
-// the N-tile (ent[j] < 0: from the zero buffer)
+// bytes per instruction, LDS base in M0): piece q = kPPG j + t fills bytes
+// [1024 t, 1024 t + 1024) of slot j from the row of K-group ek[j] >> 12
+// (ek[j] < 0: the zero row nkg), each lane from its offset nlo + 128 t
 template <int WAVES>
-__device__ __forceinline__ void stage_piece_k8(const __bf16 *Bnt, int4 ent, const __bf16 *bzero,
-                                               unsigned char *buf, int wave, int lane, int i) {
+__device__ __forceinline__ void stage_piece_k8(const __bf16 *Bset, size_t rowst, int nkg, int4 ek,
+                                               int nlo, unsigned char *buf, int wave, int lane,
+                                               int i) {
     const int piece = wave + WAVES * i;
     const int j = piece / kPPG, t = piece - kPPG * j;
-    const int v = j == 0 ? ent.x : j == 1 ? ent.y : j == 2 ? ent.z : ent.w;
-    const __bf16 *grp = v < 0 ? bzero : Bnt + (size_t)(v >> 12) * kMx2GroupElems;
-    const unsigned char *src = (const unsigned char *)grp + t * 1024 + lane * 16;
+    const int v = pick4(ek, j);
+    const __bf16 *src = Bset + (size_t)(v < 0 ? nkg : v >> 12) * rowst + 128 * t + nlo;
     if (t < kPPG - 1 || lane < kLastLanes)
         __builtin_amdgcn_global_load_lds((const void *)src,
                                          (lds_void *)(buf + j * kGroupBytes + t * 1024), 16, 0, 0);
 }
 
 template <int WAVES>
-__device__ __forceinline__ void stage_block_k8(const __bf16 *Bnt, int4 ent, const __bf16 *bzero,
-                                               unsigned char *buf, int wave, int lane) {
+__device__ __forceinline__ void stage_block_k8(const __bf16 *Bset, size_t rowst, int nkg, int4 ek,
+                                               int nlo, unsigned char *buf, int wave, int lane) {
     const int np = pieces_of(wave, WAVES);
 #pragma unroll
     for (int i = 0; i < pieces_max<WAVES>(); i++)
-        if (i < np) stage_piece_k8<WAVES>(Bnt, ent, bzero, buf, wave, lane, i);
+        if (i < np) stage_piece_k8<WAVES>(Bset, rowst, nkg, ek, nlo, buf, wave, lane, i);
 }
 
 // One step of a transposing sum over the 16 lanes of a DPP row: lanes whose
@@ -196,13 +219,11 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
     if (rtile >= a.rows / kRows) return;  // padding of the last run
     const size_t row0 = (size_t)rtile * kRows;
     const int lda = a.lda;
-    const int nks = lda / 32;
     const size_t rows = (size_t)a.rows;
-    const int *L = a.blist + (size_t)rtile * a.nblk;
     const int na = a.bcount[rtile];  // listed blocks
-    // K-steps of four live 8-function groups of the listed blocks: entry =
-    // (global group << 12) | position of the group in this tile's
-    // compressed rows; -1 pads the last step (zero right operand)
+    // N-tiles and K-steps of four live 8-function groups: entry = (global
+    // group << 12) | position of the group in this tile's compressed rows;
+    // -1 pads the last four (zero right operand)
     const int *K8 = a.k8list + (size_t)rtile * a.nblk * 4;
     const int nq = (a.k8count[rtile] + 3) >> 2;
     // This tile's basis rows: dense [row][lda], or compressed to the listed
@@ -233,13 +254,17 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
         const int x = g == 0 ? v.x : g == 1 ? v.y : g == 2 ? v.z : v.w;
         return x < 0 ? 0 : (size_t)(cmp ? (x & 4095) : (x >> 12)) * (8 * kAParts);
     };
-    const __bf16 *Bset = a.bs3 + (size_t)set * a.ntiles * nks * kMx2StepElems;
-    const int e0 = (int)((long long)split * na / a.nsplit);
-    const int e1 = (int)((long long)(split + 1) * na / a.nsplit);
+    // pair blocks of this density set: rows of K-groups (nkg: the zero row)
+    // of ngn + 1 N-groups (ngn: the zero block)
+    const int nkg = lda / 8, ngn = a.ncol_pad / 8;
+    const size_t rowst = (size_t)(ngn + 1) * kMx2PairElems;
+    const __bf16 *Bset = a.bs3 + (size_t)set * mx2_set_elems(lda, a.ncol_pad);
+    const int e0 = (int)((long long)split * nq / a.nsplit);
+    const int e1 = (int)((long long)(split + 1) * nq / a.nsplit);
 
     for (int e = e0; e < e1; e++) {
-        const int nt = L[e];
-        const __bf16 *Bnt = Bset + (size_t)nt * nks * kMx2StepElems;
+        const int4 en = *(const int4 *)(K8 + 4 * e);  // this N-tile's four groups
+        const int nlo = lane_offset(en, ngn, lane);
         v4f acc[kRB][kCB];
 #pragma unroll
         for (int rb = 0; rb < kRB; rb++)
@@ -247,8 +272,9 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
             for (int cb = 0; cb < kCB; cb++) acc[rb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
 
         const int4 q0 = *(const int4 *)K8, q1 = *(const int4 *)(K8 + (nq > 1 ? 4 : 0));
-        stage_block_k8<WAVES>(Bnt, q0, a.bzero, bbuf, wave, lane);
-        if (NBUF == 3 && nq > 1) stage_block_k8<WAVES>(Bnt, q1, a.bzero, bbuf + kBufBytes, wave, lane);
+        stage_block_k8<WAVES>(Bset, rowst, nkg, q0, nlo, bbuf, wave, lane);
+        if (NBUF == 3 && nq > 1)
+            stage_block_k8<WAVES>(Bset, rowst, nkg, q1, nlo, bbuf + kBufBytes, wave, lane);
         v8bf an[kRB][kAParts];
         {
             const size_t ao = a_off(q0);
@@ -317,7 +343,7 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
                         an[j / kAParts][j % kAParts] = *(const v8bf *)(
                             Arow + (j / kAParts) * 16 * arow + aoff + (j % kAParts) * 8);
                 if (copy && cb >= kCopyFrom && cb - kCopyFrom < np)
-                    stage_piece_k8<WAVES>(Bnt, ec, a.bzero, cbuf, wave, lane, cb - kCopyFrom);
+                    stage_piece_k8<WAVES>(Bset, rowst, nkg, ec, nlo, cbuf, wave, lane, cb - kCopyFrom);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int rb = 0; rb < kRB; rb++) acc[rb][cb] = mfma_split(as[rb], bf[sl], acc[rb][cb]);
@@ -347,14 +373,20 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
         }
 
         // epilogue: lane (g, c) holds Y[row 4g + r][col c] of every block;
-        // column block cb = 2 * matrix + half covers functions nt*32 + 16*half + c.
+        // column block cb = 2 * matrix + half covers function c & 7 of the
+        // N-tile's group 2 * half + (c >> 3) (padding: a zero column).
         // The global loads (phi parts and gradients) of each row are issued
         // one row ahead, so their latency overlaps the previous row's work.
         float R[2][3];
+        int acol[2];  // column of the lane's function in this tile's basis rows
 #pragma unroll
-        for (int h = 0; h < 2; h++)
+        for (int h = 0; h < 2; h++) {
+            const int x = pick4(en, 2 * h + (c >> 3));
+            const int fn = x < 0 ? 0 : (x >> 12) * 8 + (c & 7);
+            acol[h] = x < 0 ? 0 : (cmp ? (x & 4095) : (x >> 12)) * 8 + (c & 7);
 #pragma unroll
-            for (int d = 0; d < 3; d++) R[h][d] = a.rf32[(size_t)d * a.ncol_pad + nt * kNu + 16 * h + c];
+            for (int d = 0; d < 3; d++) R[h][d] = a.rf32[(size_t)d * a.ncol_pad + fn];
+        }
         constexpr int kIt = 4 * kRB;  // rows per lane: it = 4 rb + r
         auto row_of = [&](int it) { return wave * 16 * kRB + (it >> 2) * 16 + 4 * g + (it & 3); };
         __bf16 pc[2][kAParts], pn[2][kAParts];
@@ -363,12 +395,10 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
             const size_t rl = row_of(it);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                // column of function nt*32 + 16h + c in this tile's basis rows
-                const int acol = (cmp ? e : nt) * kNu + 16 * h + c;
-                const __bf16 *pe = Atile + rl * arow + (acol >> 3) * (8 * kAParts) + (acol & 7);
+                const __bf16 *pe = Atile + rl * arow + (acol[h] >> 3) * (8 * kAParts) + (acol[h] & 7);
 #pragma unroll
                 for (int k = 0; k < kAParts; k++) pv[h][k] = pe[8 * k];
-                const size_t gi = rl * wcols + acol;
+                const size_t gi = rl * wcols + acol[h];
 #pragma unroll
                 for (int d = 0; d < 3; d++) Gv[h][d] = Gtile[d * gpl + gi];
             }

@@ -119,7 +119,7 @@ struct ContractArgs {
     const double *a64, *g64, *b64;  // FP64 path
     const float *g32;    // mixed path: FP32 gradients
     const __bf16 *as3;   // split basis values [rows][lda/8][kAParts][8] (mixed kernels)
-    const __bf16 *bs3;   // split right operand, LDS-image blocks per (set, N-tile, K-step)
+    const __bf16 *bs3;   // split right operand, pair blocks per (set, K-group, N-group)
     const double *rf64;  // [3][ncol_pad] atom coordinate of each function
     const float *rf32;
     double *part;        // [nmatset][nsplit][kNQ][rows]; pre-split kernels: [nmatset][nsplit][rows][kNQ]
@@ -128,8 +128,9 @@ struct ContractArgs {
     // a tile takes the listed N-blocks [s * count / nsplit, (s+1) * count / nsplit)
     const int *bcount, *blist;
     int nblk;
-    // pre-split kernels: K-steps of four listed 8-function groups (TileArgs)
-    // and a zeroed 10752-byte buffer standing in for the padding groups
+    // pre-split kernels: N-tiles and K-steps of four live 8-function groups
+    // (TileArgs); the FP64 kernel's K-steps too, with a zeroed 12 KiB
+    // buffer standing in for the padding groups
     const int *k8count, *k8list;
     const __bf16 *bzero;
     // compressed operands (pre-split kernels; see EvalArgs): null for dense
@@ -156,13 +157,20 @@ constexpr int split_products() {
     return n;
 }
 constexpr int kMx2Cols = 224;                             // 7 matrices x 32 functions
-// right operand in HBM: per (density set, N-tile, 8-function group kg) one
-// group of kBParts parts x 224 columns x 8, [part][column][8]; the four
-// groups of a 32-function K-block are consecutive (a K-step image,
-// [group][part][column][8]), plus 2 KiB of slack at the end
+// right operand in HBM: one pair block per (density set, 8-function K-group
+// gk, 8-function N-group gn) of kBParts parts x 7 matrices x 8 N-functions x
+// 8 K-functions, [part][matrix][n][k]; K-groups major, each a row of
+// ncol_pad / 8 + 1 pair blocks whose last is zero, and a zero row last
+// (lda / 8 + 1 rows), so that the padding entries of the group lists
+// (-1) address zeros
+constexpr int kMx2PairElems = kBParts * kNMat * 64;      // 1344 for three parts
+__host__ __device__ constexpr size_t mx2_set_elems(int lda, int ncol_pad) {
+    return (size_t)(lda / 8 + 1) * (size_t)(ncol_pad / 8 + 1) * kMx2PairElems;
+}
+// the kernel's LDS buffer of one K-step: four K-groups (slots), each
+// kBParts parts x 224 columns x 8, [slot][part][column][8], column = matrix
+// * 32 + 8 * (N-group of the N-tile) + function; in whole KiB
 constexpr int kMx2GroupElems = kBParts * kMx2Cols * 8;   // 5376 for three parts
-constexpr int kMx2StepElems = 4 * kMx2GroupElems;        // 21504
-// the kernel's LDS buffer of one K-step: four groups in whole KiB
 constexpr int kMx2BlockElems = (4 * kMx2GroupElems * 2 + 1023) / 1024 * 512;
 
 struct FinalizeArgs {

@@ -321,14 +321,17 @@ class Context:
         check(self._lib.gimic_ctx_set_tile_stats(self._h, 1 if on else 0))
 
     def tile_stats(self):
-        """dict(tiles, blocks, blocks_sq, tile_rows, nblk, blocks_kn): row
-        tiles seen, sum of listed blocks and of their squares, rows per tile,
-        blocks of a dense row, and the sum of listed blocks x live
-        8-function groups / 4 (the pre-split kernels' block products)"""
-        out = np.zeros(6)
+        """dict(tiles, blocks, blocks_sq, tile_rows, nblk, blocks_kn,
+        groups_sq): row tiles seen, sum of listed blocks and of their
+        squares, rows per tile, blocks of a dense row, the sum of listed
+        blocks x live 8-function groups / 4 (the FP64 kernel's block
+        products) and of squared live groups (the pre-split mixed kernels'
+        group products)"""
+        out = np.zeros(7)
         check(self._lib.gimic_ctx_tile_stats(self._h, ptr(out)))
         return dict(tiles=int(out[0]), blocks=float(out[1]), blocks_sq=float(out[2]),
-                    tile_rows=int(out[3]), nblk=int(out[4]), blocks_kn=float(out[5]))
+                    tile_rows=int(out[3]), nblk=int(out[4]), blocks_kn=float(out[5]),
+                    groups_sq=float(out[6]))
 
     def set_graph(self, enable=True):
         """capture flux_planes batches into one reusable hipGraph launch"""

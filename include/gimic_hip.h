@@ -185,13 +185,14 @@ int gimic_ctx_chunk_bytes(const gimic_ctx *ctx, int64_t *bytes);
  * enabled (on != 0; enabling clears the totals), every eager evaluation
  * that runs a block-list kernel copies its row tiles' block counts to the
  * host (a synchronising copy: use it outside timed regions) and adds them
- * up.  out[6]: row tiles, sum of listed blocks, sum of squared listed
- * blocks, rows per tile, blocks per dense row (lda / 32), and the sum over
- * tiles of listed blocks x live 8-function groups / 4.  A tile of R rows
- * with n listed blocks and k live groups costs 2 * R * 7 * 32 n * 8 k
- * flops per density set (the FP64 and pre-split mixed kernels step K
- * through the live groups; a kernel that walked whole blocks would do
- * 2 * R * 7 * (32 n)^2). */
+ * up.  out[7]: row tiles, sum of listed blocks, sum of squared listed
+ * blocks, rows per tile, blocks per dense row (lda / 32), the sum over
+ * tiles of listed blocks x live 8-function groups / 4, and the sum over
+ * tiles of squared live groups.  A tile of R rows with k live groups
+ * costs 2 * R * 7 * (8 k)^2 flops per density set (the FP64 and pre-split
+ * mixed kernels walk the live groups along both N and K; with n listed
+ * blocks, N-tiles of whole blocks would do 2 * R * 7 * 32 n * 8 k, and a
+ * kernel that walked whole blocks both ways 2 * R * 7 * (32 n)^2). */
 int gimic_ctx_set_tile_stats(gimic_ctx *ctx, int on);
 int gimic_ctx_tile_stats(const gimic_ctx *ctx, double *out);
 

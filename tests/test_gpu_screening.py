@@ -108,6 +108,9 @@ def test_lists_leave_out_blocks_and_fp64_matches_oracle(cage):
     assert st["tile_rows"] == 128 and st["tiles"] == len(pts) // 128
     frac = st["blocks"] / (st["tiles"] * st["nblk"])
     assert frac < 0.8, frac  # most of the far side's blocks are screened
+    # the live groups are within the listed blocks: k <= 4 n on every tile,
+    # so sum k^2 <= 16 sum n^2
+    assert 0 < st["groups_sq"] <= 16 * st["blocks_sq"]
     sub = np.random.default_rng(5).choice(len(pts), 48, replace=False)
     dens = sph.to_cartesian(s.dens, s.shell_l)  # the oracle is cartesian: D_cart = T^T D T
     ref = oracle.jtensor(s.arrays(), dens, pts[sub], screening=1e-8)
@@ -124,7 +127,7 @@ def test_points_beyond_every_radius_give_zero(cage):
         assert np.all(J == 0.0)
     st = ctx.tile_stats()
     ctx.set_tile_stats(False)
-    assert st["tiles"] > 0 and st["blocks"] == 0
+    assert st["tiles"] > 0 and st["blocks"] == 0 and st["groups_sq"] == 0
 
 
 def test_unscreened_lists_are_full():
@@ -135,7 +138,10 @@ def test_unscreened_lists_are_full():
         ctx.set_tile_stats(True)
         J = ctx.jtensor(pts, precision="fp64")
         st = ctx.tile_stats()
+        nbf = ctx.nbf
     assert st["blocks"] == st["tiles"] * st["nblk"]
+    # every 8-function group holding a function is live on every tile
+    assert st["groups_sq"] == st["tiles"] * ((nbf + 7) // 8) ** 2
     dens = sph.to_cartesian(s.dens, s.shell_l)
     ref = oracle.jtensor(s.arrays(), dens, pts[:40], screening=None)
     assert rel_err(J[:40], ref).max() <= 1e-10
