@@ -832,8 +832,6 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         ca.bs3 = M.bs3.as<__bf16>();
         ca.rf64 = c->rf64.as<double>();
         ca.rf32 = c->rf32.as<float>();
-        ca.bcount = W.bcount.as<int>();
-        ca.blist = W.blist.as<int>();
         ca.nblk = nblk;
         ca.k8count = W.k8count.as<int>();
         ca.k8list = W.k8list.as<int>();
@@ -898,7 +896,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             else plan(mixed_tile_rows(variant), kMx2Cols / kNMat, split);
         };
         // mixed kernels: row-major partial sums added into a cleared buffer,
-        // and basis rows compressed to each row tile's listed blocks (lists
+        // and basis rows compressed to each row tile's live groups (lists
         // and offsets built before the basis evaluation)
         auto run = [&](int variant, int split = -1) {
             plan_variant(variant, split);
@@ -921,10 +919,10 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             if (variant != 0) {
                 const int tr = mixed_tile_rows(variant);
                 lists(tr);
-                hip_check(launch_tile_offsets(W.bcount.as<int>(), rows / tr, W.toff.as<int>(), st),
+                hip_check(launch_tile_offsets(W.k8count.as<int>(), rows / tr, W.toff.as<int>(), st),
                           "tile_offsets");
-                ea.bcount = W.bcount.as<int>();
-                ea.blist = W.blist.as<int>();
+                ea.k8count = W.k8count.as<int>();
+                ea.k8list = W.k8list.as<int>();
                 ea.toff = W.toff.as<int>();
                 ea.nblk = nblk;
                 ea.tile_rows = tr;

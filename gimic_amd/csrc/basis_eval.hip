@@ -280,15 +280,16 @@ __global__ __launch_bounds__(256) void basis_eval_kernel(EvalArgs a) {
 // Split output through LDS (see the file comment).  Grid: rows / tile_points
 // workgroups; tile_points divides 128 (the smallest row tile of the
 // contraction kernels) and the LDS image is
-// tile_points * lda * 18 bytes (then, compressed, nblk shorts).
+// tile_points * lda * 18 bytes (then, compressed, 4 nblk shorts).
 //
 // Compressed output (a.toff non-null): the workgroup's rows belong to row
-// tile t of a.tile_rows rows, which keeps only its listed blocks.  A shell
-// is evaluated only if its first block is listed: a shell with a function
-// in an unlisted block lies beyond its radius for every point of the tile
-// (which is what leaving the block out means), and a shell that is not has
-// all its functions in listed blocks, adjacent in the list, so its
-// functions stay contiguous in the compressed columns.
+// tile t of a.tile_rows rows, which keeps only its live 8-function groups.
+// A shell is evaluated only if its first group is live: a shell with a
+// function in a group that is not lies beyond its radius for every point
+// of the tile (the groups of every shell within reach of the tile are
+// live), and a shell that is within reach has all its functions in live
+// groups, adjacent in the list, so its functions stay contiguous in the
+// compressed columns.
 //
 // F64: FP64 values and gradients of dense rows (the FP64 contraction's
 // input), 32 bytes per function in the image; no compression.  Staging
@@ -303,20 +304,20 @@ void basis_eval_staged_kernel(EvalArgs a) {
     const int tid = threadIdx.x, tp = a.tile_points;
     const size_t row0 = (size_t)blockIdx.x * tp;
     int wt = a.lda;  // functions per row of this tile's output
-    short *bpos = nullptr;
+    short *gpos = nullptr;  // place of each 8-function group in the compressed rows
     __bf16 *phi_dst = a.phis + row0 * kAParts * (size_t)a.lda;
     float *grad_dst = a.grad32 + row0 * (size_t)a.lda;
     size_t grad_plane = (size_t)a.gplane;
     if (!F64 && a.toff) {
         const int t = (int)(row0 / a.tile_rows);
-        const int na = a.bcount[t];
-        if (na == 0) return;  // nothing of this tile is read (uniform per workgroup)
-        wt = 32 * na;
-        bpos = (short *)(basis_tile + a.lds_image);
-        for (int i = tid; i < a.nblk; i += 256) bpos[i] = -1;
+        const int kg = a.k8count[t];
+        if (kg == 0) return;  // nothing of this tile is read (uniform per workgroup)
+        wt = 8 * kg;
+        gpos = (short *)(basis_tile + a.lds_image);
+        for (int i = tid; i < 4 * a.nblk; i += 256) gpos[i] = -1;
         __syncthreads();
-        for (int i = tid; i < na; i += 256) bpos[a.blist[(size_t)t * a.nblk + i]] = (short)i;
-        const size_t base = (size_t)a.toff[t] * a.tile_rows * 32;
+        for (int i = tid; i < kg; i += 256) gpos[a.k8list[(size_t)t * a.nblk * 4 + i] >> 12] = (short)i;
+        const size_t base = (size_t)a.toff[t] * a.tile_rows * 8;
         const size_t rin = row0 - (size_t)t * a.tile_rows;
         phi_dst = a.phis + base * kAParts + rin * kAParts * (size_t)wt;
         grad_dst = a.grad32 + base * 3 + rin * (size_t)wt;
@@ -324,12 +325,12 @@ void basis_eval_staged_kernel(EvalArgs a) {
     }
     // Candidate shells (CAND): the shells that can reach a row of this workgroup,
     // listed in shell order, kCandChunk at a time.  A shell is dropped only
-    // if its block is not listed (compressed output) or if it is beyond its
+    // if its group is not live (compressed output) or if it is beyond its
     // screening radius from the rows' bounding sphere, hence from every row;
     // the per-row radius test below still decides each (row, shell) pair,
     // so the output is the same, but the lanes of a wavefront no longer
     // idle beside pairs that every row screens out.
-    short *cand = (short *)(basis_tile + a.lds_image + ((size_t)a.nblk * 2 + 15) / 16 * 16);
+    short *cand = (short *)(basis_tile + a.lds_image + ((size_t)a.nblk * 8 + 15) / 16 * 16);
     double *sphere = (double *)(cand + kCandChunk);  // centre x, y, z and radius
     int *wcnt = (int *)(sphere + 4);                 // kept shells per wavefront
     if (CAND) {
@@ -374,7 +375,7 @@ void basis_eval_staged_kernel(EvalArgs a) {
             bool keep = false;
             if (i0 + tid < kCandChunk && si < a.nshell && sphere[3] >= 0.0) {
                 const DevShell sh = a.shells[si];
-                keep = !bpos || bpos[sh.fout >> 5] >= 0;
+                keep = !gpos || gpos[sh.fout >> 3] >= 0;
                 if (keep) {
                     const double *R = a.coords + 3 * sh.atom;
                     const double dx = sphere[0] - R[0], dy = sphere[1] - R[1], dz = sphere[2] - R[2];
@@ -423,10 +424,10 @@ void basis_eval_staged_kernel(EvalArgs a) {
             const size_t row = row0 + r0 + lp;
             if (row >= (size_t)a.npts) return;  // padding rows stay zero
             DevShell sh = a.shells[si];
-            if (bpos) {
-                const int cb = bpos[sh.fout >> 5];
-                if (cb < 0) return;  // zero over the whole tile
-                sh.fout = cb * 32 + (sh.fout & 31);
+            if (gpos) {
+                const int cg = gpos[sh.fout >> 3];
+                if (cg < 0) return;  // zero over the whole tile
+                sh.fout = cg * 8 + (sh.fout & 7);
             }
             const double *r = a.pts + 3 * row;
             const double *R = a.coords + 3 * sh.atom;
@@ -539,10 +540,10 @@ hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s) {
         if (row_bytes > 150 * 1024) return hipErrorInvalidValue;
         b.tile_points = tp;
         b.lds_image = image;
-        // the image, the block positions (16-byte aligned) and, for bases of
+        // the image, the group positions (16-byte aligned) and, for bases of
         // kCandMinShells shells or more, the candidate list
         const bool cand = a.nshell >= kCandMinShells;
-        const size_t lds = image + ((size_t)a.nblk * 2 + 15) / 16 * 16 + (cand ? kCandBytes : 0);
+        const size_t lds = image + ((size_t)a.nblk * 8 + 15) / 16 * 16 + (cand ? kCandBytes : 0);
         // (the candidate list's bounding sphere takes one row per lane of a wavefront)
         if (a.rows % tp || tp > 64 || a.lmax < 0 || a.lmax > kDevMaxL) return hipErrorInvalidValue;
         if (a.toff && (a.tile_rows % tp || a.rows % a.tile_rows || a.nblk * 32 != a.lda))

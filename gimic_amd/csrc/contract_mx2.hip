@@ -220,21 +220,20 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
     const size_t row0 = (size_t)rtile * kRows;
     const int lda = a.lda;
     const size_t rows = (size_t)a.rows;
-    const int na = a.bcount[rtile];  // listed blocks
     // N-tiles and K-steps of four live 8-function groups: entry = (global
     // group << 12) | position of the group in this tile's compressed rows;
     // -1 pads the last four (zero right operand)
     const int *K8 = a.k8list + (size_t)rtile * a.nblk * 4;
     const int nq = (a.k8count[rtile] + 3) >> 2;
-    // This tile's basis rows: dense [row][lda], or compressed to the listed
-    // blocks (a.toff non-null: the basis kernel wrote only those, so A and
+    // This tile's basis rows: dense [row][lda], or compressed to the live
+    // groups (a.toff non-null: the basis kernel wrote only those, so A and
     // the gradients are addressed by list position)
     const bool cmp = a.toff != nullptr;
-    const size_t wcols = cmp ? (size_t)kNu * na : (size_t)lda;  // functions per row
-    const size_t arow = kAParts * wcols;                        // BF16 per split row
-    const __bf16 *Atile = cmp ? a.as3 + (size_t)a.toff[rtile] * kRows * (kNu * kAParts)
+    const size_t wcols = cmp ? (size_t)8 * a.k8count[rtile] : (size_t)lda;  // functions per row
+    const size_t arow = kAParts * wcols;                                     // BF16 per split row
+    const __bf16 *Atile = cmp ? a.as3 + (size_t)a.toff[rtile] * kRows * (8 * kAParts)
                               : a.as3 + row0 * arow;
-    const float *Gtile = cmp ? a.g32 + (size_t)a.toff[rtile] * kRows * 96 : a.g32 + row0 * wcols;
+    const float *Gtile = cmp ? a.g32 + (size_t)a.toff[rtile] * kRows * 24 : a.g32 + row0 * wcols;
     const size_t gpl = cmp ? (size_t)kRows * wcols : (size_t)a.gplane;  // gradient plane stride
     const int np = pieces_of(wave, WAVES);  // pieces this wavefront copies per block
     // this workgroup's partial sums, row-major [row][kNQ]

@@ -92,18 +92,19 @@ struct EvalArgs {
     double *phi64, *grad64;  // [rows][lda], [3][rows][lda]
     // mixed outputs: FP32 gradients, and each value x as kAParts BF16 parts
     // (x = h + m + l exactly with three), layout [row][lda/8][kAParts][8]
-    // (compressed to the listed blocks of each row tile, below)
+    // (compressed to the live groups of each row tile, below)
     float *grad32;
     __bf16 *phis;
     int tile_points;     // points per workgroup of the LDS-staged kernel (set by the launcher)
     size_t lds_image;    // bytes of its LDS image (set by the launcher)
     int lmax;            // largest shell l of the basis
     // Compressed split output (staged kernel, toff non-null): a row tile of
-    // tile_rows rows keeps only its bcount[t] listed 32-function blocks,
-    // blist[t * nblk ...]; its rows start at element toff[t] * tile_rows *
-    // 32 kAParts of phis ([row][32 n / 8][kAParts][8]) and toff[t] *
-    // tile_rows * 96 of grad32 ([3][tile_rows][32 n])
-    const int *bcount, *blist, *toff;
+    // tile_rows rows keeps only its k8count[t] live 8-function groups
+    // (k8list[t * 4 nblk ...], global group << 12 | place k in the row); its
+    // rows start at element toff[t] * tile_rows * 8 kAParts of phis
+    // ([row][k][kAParts][8]) and toff[t] * tile_rows * 24 of grad32
+    // ([3][tile_rows][8 k])
+    const int *k8count, *k8list, *toff;
     int nblk, tile_rows;
 };
 hipError_t launch_basis_eval(const EvalArgs &a, int mode, hipStream_t s);
@@ -123,14 +124,12 @@ struct ContractArgs {
     const double *rf64;  // [3][ncol_pad] atom coordinate of each function
     const float *rf32;
     double *part;        // [nmatset][nsplit][kNQ][rows]; pre-split kernels: [nmatset][nsplit][rows][kNQ]
-    // screening block lists per row tile (kernels that walk them): bcount[t]
-    // 32-function blocks, listed ascending in blist[t * nblk ...]; split s of
-    // a tile takes the listed N-blocks [s * count / nsplit, (s+1) * count / nsplit)
-    const int *bcount, *blist;
+    // screening lists per row tile (TileArgs): the live 8-function groups of
+    // the listed 32-function blocks, k8count[t] of them in k8list[t * 4 nblk
+    // ...]; the pre-split kernels take four per N-tile and K-step, the FP64
+    // kernel two (a zeroed 12 KiB buffer stands in for its padding groups);
+    // split s of a tile takes N-tiles [s * count / nsplit, (s+1) * count / nsplit)
     int nblk;
-    // pre-split kernels: N-tiles and K-steps of four live 8-function groups
-    // (TileArgs); the FP64 kernel's K-steps too, with a zeroed 12 KiB
-    // buffer standing in for the padding groups
     const int *k8count, *k8list;
     const __bf16 *bzero;
     // compressed operands (pre-split kernels; see EvalArgs): null for dense
@@ -251,13 +250,13 @@ struct TileArgs {
     int *bcount, *blist;  // [rows / tile_rows], [rows / tile_rows][nblk]
     // 8-function groups of the listed blocks that screening leaves non-zero:
     // k8count[t] groups, k8list[t * 4 nblk ...] = (global group << 12) |
-    // (4 * list position of its block + group within the block), padded
+    // (place k of the group in this list: its place in the compressed rows), padded
     // with -1 to a multiple of 4 (nblk <= 1024)
     int *k8count, *k8list;
 };
 hipError_t launch_block_lists(const TileArgs &a, hipStream_t s);
-// toff[t] = sum of bcount[0 .. t-1] for t = 0 .. ntiles (one workgroup)
-hipError_t launch_tile_offsets(const int *bcount, int ntiles, int *toff, hipStream_t s);
+// toff[t] = sum of cnt[0 .. t-1] for t = 0 .. ntiles (one workgroup)
+hipError_t launch_tile_offsets(const int *cnt, int ntiles, int *toff, hipStream_t s);
 
 hipError_t run_mfma_layout_probe(double *h_f64, float *h_bf16);
 hipError_t run_mfma32_layout_probe(float *h_out);  // 2048 floats

@@ -273,20 +273,22 @@ __global__ __launch_bounds__(256) void block_list_kernel(TileArgs a) {
     __syncthreads();
     if (t == 0) {
         int *kl = a.k8list + (long long)blockIdx.x * a.nblk * 4;
-        int k = 0, pos = 0;
+        int k = 0;
         for (int b = 0; b < a.nblk; b++) {
             const unsigned m = gmask[b];
             if (!m) continue;
             for (int gq = 0; gq < 4; gq++)
-                if (m >> gq & 1u) kl[k++] = ((4 * b + gq) << 12) | (4 * pos + gq);
-            pos++;
+                if (m >> gq & 1u) {
+                    kl[k] = ((4 * b + gq) << 12) | k;
+                    k++;
+                }
         }
         a.k8count[blockIdx.x] = k;
         while (k & 3) kl[k++] = -1;
     }
 }
 
-// exclusive prefix sum of the tiles' block counts, toff[0 .. n]
+// exclusive prefix sum of the tiles' live group counts, toff[0 .. n]
 __global__ __launch_bounds__(256) void tile_offsets_kernel(const int *cnt, int n, int *toff) {
     __shared__ int s[256];
     const int t = threadIdx.x;
@@ -312,9 +314,9 @@ __global__ __launch_bounds__(256) void tile_offsets_kernel(const int *cnt, int n
 
 }  // namespace
 
-hipError_t launch_tile_offsets(const int *bcount, int ntiles, int *toff, hipStream_t s) {
+hipError_t launch_tile_offsets(const int *cnt, int ntiles, int *toff, hipStream_t s) {
     if (ntiles <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(tile_offsets_kernel, dim3(1), dim3(256), 0, s, bcount, ntiles, toff);
+    hipLaunchKernelGGL(tile_offsets_kernel, dim3(1), dim3(256), 0, s, cnt, ntiles, toff);
     return hipGetLastError();
 }
 
