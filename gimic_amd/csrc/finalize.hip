@@ -23,11 +23,22 @@ __device__ __forceinline__ double eps3(int i, int j, int k) {
 }
 
 __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
+    // row-major partials: the workgroup's 256 rows are 256 kNQ contiguous
+    // doubles per (set, split), staged through LDS with coalesced 16-byte
+    // loads (a thread's own row is 176 bytes: read directly, every load of
+    // a wavefront would touch 64 separate pieces)
+    __shared__ double2 stage[256 * kNQ / 2];
     const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= a.npts) return;
-    const double r[3] = {a.pts[3 * p], a.pts[3 * p + 1], a.pts[3 * p + 2]};
-    long long o = a.perm ? a.perm[p] : p;  // caller index of row p
-    if (a.pidx) o = a.pidx[o];             // its place in the output arrays
+    const bool valid = p < a.npts;
+    double r[3] = {0.0, 0.0, 0.0};
+    long long o = 0;
+    if (valid) {
+        r[0] = a.pts[3 * p];
+        r[1] = a.pts[3 * p + 1];
+        r[2] = a.pts[3 * p + 2];
+        o = a.perm ? a.perm[p] : p;  // caller index of row p
+        if (a.pidx) o = a.pidx[o];   // its place in the output arrays
+    }
     for (int set = 0; set < a.nmatset; set++) {
         double q[kNQ];
 #pragma unroll
@@ -35,19 +46,22 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
         for (int s = 0; s < a.nsplit; s++) {
             const double *src = a.part + ((size_t)set * a.nsplit + s) * kNQ * (size_t)a.rows;
             if (a.part_rows) {
-                // a row's kNQ sums are 176 contiguous, 16-byte aligned bytes
-                const double2 *v = (const double2 *)(src + (size_t)p * kNQ);
+                const double2 *blk = (const double2 *)(src + (size_t)blockIdx.x * 256 * kNQ);
+                __syncthreads();  // the previous (set, split) has been read out
+                for (int i = threadIdx.x; i < 256 * kNQ / 2; i += 256) stage[i] = blk[i];
+                __syncthreads();
 #pragma unroll
                 for (int j = 0; j < kNQ / 2; j++) {
-                    const double2 t = v[j];
+                    const double2 t = stage[threadIdx.x * (kNQ / 2) + j];
                     q[2 * j] += t.x;
                     q[2 * j + 1] += t.y;
                 }
-            } else {
+            } else if (valid) {
 #pragma unroll
                 for (int i = 0; i < kNQ; i++) q[i] += src[(size_t)i * a.rows + p];
             }
         }
+        if (!valid) continue;
         // q[1+c] = E_c = rho r_c - C_c, C_c = sum_nu Y0 phi R_nu,c.  With all
         // three terms on, eps_bmd C_d + eps_mbc rho r_c = eps_mbd E_d, so the
         // tensor is assembled without forming rho r and C separately.
@@ -241,6 +255,8 @@ hipError_t launch_fill_zero(double *p, size_t n, hipStream_t s) {
 
 hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s) {
     if (a.npts <= 0) return hipSuccess;
+    // row-major partials are read in whole workgroups of 256 rows
+    if (a.part_rows && (a.rows % 256 || a.npts > a.rows)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(finalize_kernel, dim3((a.npts + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
