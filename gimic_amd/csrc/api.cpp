@@ -895,14 +895,11 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
             if (variant == 0) plan(kWgRows, kNT);
             else plan(mixed_tile_rows(variant), kMx2Cols / kNMat, split);
         };
-        // mixed kernels: row-major partial sums added into a cleared buffer,
+        // mixed kernels: row-major partial sums (written whole by the launch),
         // and basis rows compressed to each row tile's live groups (lists
         // and offsets built before the basis evaluation)
         auto run = [&](int variant, int split = -1) {
             plan_variant(variant, split);
-            if (variant != 0)
-                hip_check(launch_fill_zero(ca.part, (size_t)nmatset * ca.nsplit * kNQ * rows, st),
-                          "fill_zero");
             if (variant == 0) lists(kWgRows);  // the FP64 kernel walks lists over dense rows
             ca.toff = variant != 0 ? W.toff.as<int>() : nullptr;
             if (variant == 0) hip_check(launch_contract_f64(ca, st), "contract_f64");

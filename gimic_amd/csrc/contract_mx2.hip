@@ -32,13 +32,15 @@
 // load, and each staged block feeds twice the rows.
 //
 // Per-point partial sums.  After the butterfly over the 16 columns of a lane
-// group every lane holds all kNQ sums of its row; lane c adds sums c and
-// c + 16 (FP64) to the row-major partial buffer [set][split][row][kNQ] with
-// no-return atomic adds, 16 lanes of a group covering 128 contiguous bytes.
-// Each address belongs to one lane of one workgroup, which adds its N-tiles
-// in order to a buffer cleared before the launch, so the sums are the same,
-// bit for bit, on every run, and no LDS holds them (three K-step buffers fit
-// with 256-row tiles).
+// group every lane holds all kNQ sums of its row; lane c writes sums c and
+// c + 16 (FP64) to the row-major partial buffer [set][split][row][kNQ],
+// 16 lanes of a group covering 128 contiguous bytes: an atomic store for
+// the workgroup's first N-tile, no-return atomic adds for the later ones
+// (one thread's atomics on one address keep program order), zeros from a
+// workgroup without N-tiles.  Each address belongs to one lane of one
+// workgroup, which adds its N-tiles in order, so the sums are the same, bit
+// for bit, on every run, no clearing pass is needed, and no LDS holds them
+// (three K-step buffers fit with 256-row tiles).
 
 //
 // Screening.  A workgroup visits only the live 8-function groups of its row
@@ -444,8 +446,15 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
                 row16_xsum(v, c);
                 const float s0 = v[0], s1 = v[1];
                 double *o = pout + (row0 + rl) * kNQ;
-                unsafeAtomicAdd(o + c, (double)s0);
-                if (c + 16 < kNQ) unsafeAtomicAdd(o + c + 16, (double)s1);
+                if (e == e0) {
+                    __hip_atomic_store(o + c, (double)s0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (c + 16 < kNQ)
+                        __hip_atomic_store(o + c + 16, (double)s1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    unsafeAtomicAdd(o + c, (double)s0);
+                    if (c + 16 < kNQ) unsafeAtomicAdd(o + c + 16, (double)s1);
+                }
             }
 #pragma unroll
             for (int h = 0; h < 2; h++) {
@@ -457,6 +466,9 @@ __global__ __launch_bounds__(64 * WAVES, 1) void contract_mx2_kernel(ContractArg
             __builtin_amdgcn_sched_barrier(0);
         }
     }
+    // a workgroup without N-tiles still owns its rows' sums for this split
+    if (e0 == e1)
+        for (int i = tid; i < kRows * kNQ; i += 64 * WAVES) pout[row0 * kNQ + i] = 0.0;
 }
 
 template <int NBUF, int KRB, int WAVES>

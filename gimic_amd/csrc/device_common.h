@@ -137,11 +137,11 @@ struct ContractArgs {
 };
 hipError_t launch_contract_f64(const ContractArgs &a, hipStream_t s);
 // Pre-split variant: both operands arrive as BF16 parts, the right operand in
-// blocks that are byte images of the kernel's LDS buffer (see contract_mx2.hip)
-// nbuf 2 or 3 LDS buffers, krb 2 or 4 row blocks per wavefront, 4 or 8
-// wavefronts (16 * krb * waves rows per workgroup: 4 waves with krb 2, nbuf
-// 2 or 3, or krb 4, nbuf 2; 8 waves with krb 2, nbuf 2 or 3).  Adds its
-// partial sums to a.part, row-major, which must be zero before the launch.
+// pair blocks that the kernel gathers into its LDS buffers (see
+// contract_mx2.hip); nbuf 2 or 3 LDS buffers, krb 2 or 4 row blocks per
+// wavefront, 4 or 8 wavefronts (16 * krb * waves rows per workgroup: 4 waves
+// with krb 2, nbuf 2 or 3, or krb 4, nbuf 2; 8 waves with krb 2, nbuf 2 or
+// 3).  Writes every partial sum of a.part, row-major (no clearing needed).
 hipError_t launch_contract_mx2(const ContractArgs &a, int nbuf, int krb, int waves, hipStream_t s);
 // Mixed arithmetic: an FP32 operand value x is the sum of BF16 parts
 // x = h + m + l (three parts: exact); the basis values (left operand) keep
