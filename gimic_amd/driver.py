@@ -21,7 +21,7 @@ first); mol.xyz (write_xyz) and grid.xyz (plot_grid_xyz) in both modes.
 All integrals come from one tensor evaluation per spin case (the reference
 evaluates J once per integral).
 
-    python -m gimic_amd.driver gimic.inp [--precision fp64|mixed]
+    python -m gimic_amd.driver gimic.inp [--precision fp64|mixed] [--dryrun]
     python -m gimic_amd.driver - < keyword-stream     (as gimic.bin is run)
 
 The input may be a gimic.inp or the keyword stream the reference's frontend
@@ -39,12 +39,15 @@ from .lib import Context
 
 
 def setup(kw, basedir="."):
+    """basis, density, grid and field of a run (gimic.F90:140-165); a dry
+    run reads no density"""
     kw = inp.with_defaults(kw)
     adv = kw["Advanced"]
     thrs = float(adv["screening_thrs"]) if adv.get("screening", False) else -1.0
     uhf = bool(kw.get("openshell", False))
     path = lambda f: f if os.path.isabs(f) else os.path.join(basedir, f)
-    ctx = Context.from_files(path(kw["basis"]), path(kw["xdens"]), uhf=uhf, screening=thrs)
+    dens = None if kw.get("dryrun", False) else path(kw["xdens"])
+    ctx = Context.from_files(path(kw["basis"]), dens, uhf=uhf, screening=thrs)
     ctx.set_flags(adv["GIAO"], adv["diamag"], adv["paramag"])
     coords, _ = ctx.atoms()
     gkw = dict(kw["Grid"])
@@ -66,6 +69,18 @@ def setup_report(g, uhf, out=None):
         print(line, file=out)
     print(" INFO: Open-shell calculation" if uhf else " INFO: Closed-shell calculation", file=out)
     print("", file=out)
+
+
+def dry_run_report(calc, out=None):
+    """what a dry run prints instead of calculating (gimic.F90:174-177, and
+    the headers run_cdens and run_integral print before they return,
+    gimic.F90:200-204, 226-230)"""
+    out = sys.stdout if out is None else out
+    print(" *** Dry run, not calculating ...", file=out)
+    print("", file=out)
+    head = "Integrating current density" if calc == "integral" else "Calculating current density"
+    print(" " + head, file=out)
+    print(" " + "*" * 41, file=out)
 
 
 def spin_cases(uhf):
@@ -258,18 +273,24 @@ def main(argv=None):
     ap.add_argument("input", nargs="?", default="gimic.inp",
                     help="gimic.inp or keyword stream; - reads the stream from standard input")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--dryrun", action="store_true",
+                    help="set up the basis, grid and field without calculating (the frontend's -y)")
     a = ap.parse_args(argv)
     if a.input == "-":
         text, basedir = sys.stdin.read(), os.getcwd()
     else:
         text, basedir = open(a.input).read(), os.path.dirname(os.path.abspath(a.input))
     kw = inp.parse_any(text)
+    if a.dryrun:
+        kw["dryrun"] = True
     kw, ctx, g, b, uhf = setup(kw, basedir)
     coords, _ = ctx.atoms()
     writers.write_mol_xyz("mol.xyz", ctx.symbols(), coords)
     writers.write_grid_xyz("grid.xyz", g, ctx.symbols(), coords)
     setup_report(g, uhf)
-    if kw.get("calc") == "integral":
+    if kw.get("dryrun", False):
+        dry_run_report(kw.get("calc"))
+    elif kw.get("calc") == "integral":
         run_integral(kw, ctx, g, b, uhf, a.precision)
     else:
         run_cdens(kw, ctx, g, b, uhf, a.precision, outdir=".")

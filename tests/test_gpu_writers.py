@@ -130,6 +130,27 @@ def test_driver_integral_run_prints_reference_lines(tmp_path, monkeypatch, capsy
         assert np.allclose(ours, want, rtol=0, atol=1.01e-6), (ours, want)
 
 
+def test_driver_dry_run_reads_no_density(tmp_path, monkeypatch, capsys):
+    """dryrun=true (the frontend's -y): the basis, grid and field are set up,
+    mol.xyz and grid.xyz written and the set-up lines printed as in a full
+    run, then the dry-run note and the calculation's header; the density
+    file is never opened (it is removed here) and nothing is calculated"""
+    from conftest import stdout_setup_lines
+    write_fixture(str(tmp_path), "c4h4.npz")
+    (tmp_path / "XDENS").unlink()
+    d = golden("c4h4.npz")
+    text = str(d["integration_inp"])
+    print("case driver c4h4 dry run", flush=True)
+    _run_driver(tmp_path, monkeypatch, "dryrun=true\n" + text)
+    out = capsys.readouterr().out
+    assert stdout_setup_lines(out) == stdout_setup_lines(str(d["integration_stdout"]))
+    lines = out.splitlines()
+    i = lines.index(" *** Dry run, not calculating ...")
+    assert lines[i + 1:i + 4] == ["", " Integrating current density", " " + "*" * 41]
+    assert "Induced current" not in out
+    assert (tmp_path / "mol.xyz").exists() and (tmp_path / "grid.xyz").exists()
+
+
 # READGRID_INP as the reference's frontend pipes it into gimic.bin
 READGRID_STREAM = """SECT top 2 True
 ARG F KW 5
