@@ -153,29 +153,47 @@ def run_integral(kw, ctx, g, b, uhf, precision="fp64", out=None):
     return res
 
 
-def run_cdens(kw, ctx, g, b, uhf, precision="fp64", outdir=None):
+def field_lines(b):
+    """what jvector_plots prints before it writes a spin case's files
+    (jfield.f90:303-306): the field as a list-directed record, then an
+    empty string written the same way"""
+    return [" magnetic field", writers.list_record(*[float(x) for x in b]).rstrip("\n"),
+            writers.list_record("").rstrip("\n")]
+
+
+def run_cdens(kw, ctx, g, b, uhf, precision="fp64", outdir=None, out=None):
     """J at every grid point per spin case (total; UHF adds alpha, beta and
     spindens, gimic.F90:196-220); with outdir, each case's files are written
     there as jvector_plots writes them (tags '', 'alpha', 'beta',
-    'spindens').  Returns {spin: (points, 9) tensors}."""
+    'spindens'), with the reference's printout (to `out`, standard output
+    when None; the reference's single-core time estimate, jfield.f90:188-229,
+    is left out).  Returns {spin: (points, 9) tensors}."""
+    out = sys.stdout if out is None else out
     pts = g.points()
     ess = kw["Essential"]
     elements = None
-    if outdir is not None and g.mode == "file":
-        ele = os.path.join(outdir, "grid.1.ele")
-        if os.path.exists(ele):
-            elements = writers.read_elements(ele)
-        else:
-            print("not writing a vtu file, because the file grid.1.ele was not found.")
-    out = {}
+    if outdir is not None:
+        print(" Calculating current density", file=out)
+        print(" " + "*" * 41, file=out)
+        if g.mode == "file":
+            ele = os.path.join(outdir, "grid.1.ele")
+            if os.path.exists(ele):
+                elements = writers.read_elements(ele)
+    res = {}
     for spin in spin_cases(uhf):
-        out[spin] = ctx.jtensor(pts, spin=spin, precision=precision)
+        res[spin] = ctx.jtensor(pts, spin=spin, precision=precision)
         if outdir is not None:
-            writers.jvector_plots(g, out[spin], b, tag="" if spin == "total" else spin,
+            for line in field_lines(b):
+                print(line, file=out)
+            if g.mode == "file" and elements is None:
+                # jfield.f90:438, list-directed
+                print(" not writing a vtu file, because the file grid.1.ele was not found.",
+                      file=out)
+            writers.jvector_plots(g, res[spin], b, tag="" if spin == "total" else spin,
                                   jmod=bool(ess.get("jmod", False)),
                                   acid=bool(ess.get("acid", False)), outdir=outdir,
                                   elements=elements)
-    return out
+    return res
 
 
 def _fortran_a(text, w):

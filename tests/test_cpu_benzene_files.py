@@ -10,7 +10,8 @@ decide.
 * The printout of the 12 integral runs (|J| on or skipped, ACID, the radius
   bound, the field check before every integration), rebuilt by
   driver.integral_report from the values it prints, comes back byte for
-  byte around the numbers.
+  byte around the numbers; of the two field runs, the header and the
+  field block (driver.field_lines).
 * The ImageData headers of every benzene VTI file (extent, origin and
   spacing printed to 17 digits) come back byte for byte.
 * int-cdens/reference/jmod.txt, re-printed from its own moduli on our Gauss
@@ -33,10 +34,9 @@ import numpy as np
 import pytest
 
 from conftest import golden, printout_mismatches, stdout_integral_lines, stdout_setup_lines
-from gimic_amd import driver
-from test_cpu_grid_flux import stdout_blocks
-from gimic_amd import grid as gridmod, inp, writers
+from gimic_amd import driver, grid as gridmod, inp, writers
 from oracle import molio
+from test_cpu_grid_flux import stdout_blocks
 
 D = None
 
@@ -132,6 +132,21 @@ def test_integral_printout_covers_the_cases():
     assert "Integration bound set to radius    2.0000000000000000" in texts["keyword-radius"]
     assert "ACID (au) sqrt(delta J^2)" in texts["keyword-magnet"]
     assert len(INTEGRAL_TESTS) == 12
+
+
+@pytest.mark.parametrize("name", [n for n in STDOUT_TESTS if n not in INTEGRAL_TESTS])
+def test_field_run_printout_matches_benzene_stdout(mol, name):
+    """a field run's header and jvector_plots' field block (gimic.F90:200-201,
+    jfield.f90:303-306): the list-directed field vector and the blank record
+    of an empty string, byte for byte (the reference's single-core time
+    estimate between them is not printed here)"""
+    kw, g, b, mlog = build(mol, str(fixture()[f"{name}/input"]))
+    lines = str(fixture()[f"{name}/stdout"]).splitlines()
+    i = lines.index(" Calculating current density")
+    assert lines[i + 1] == " " + "*" * 41
+    j = lines.index(" magnetic field")
+    assert driver.field_lines(b) == lines[j:j + 3]
+    assert lines[j + 2] == " "
 
 
 def test_setup_lines_cover_the_keywords():

@@ -47,11 +47,20 @@ def _run_driver(tmp_path, monkeypatch, inp_text):
     driver.main([str(tmp_path / "gimic.inp")])
 
 
-def test_driver_writes_openshell_3d_files(tmp_path, monkeypatch):
+def test_driver_writes_openshell_3d_files(tmp_path, monkeypatch, capsys):
     write_fixture(str(tmp_path), "openshell.npz")
     inp_text = str(golden("openshell.npz")["threed_inp"])
     print("case driver open-shell/3d cdens", flush=True)
     _run_driver(tmp_path, monkeypatch, inp_text)
+    # the printout: the run's header, then per spin case the field block of
+    # jvector_plots (jfield.f90:303-306)
+    lines = capsys.readouterr().out.splitlines()
+    i = lines.index(" Calculating current density")
+    assert lines[i + 1] == " " + "*" * 41
+    blocks = [k for k, l in enumerate(lines) if l == " magnetic field"]
+    assert len(blocks) == 4
+    for k in blocks:
+        assert len(lines[k + 1].split()) == 3 and lines[k + 2] == " "
     for tag in ("", "alpha", "beta", "spindens"):
         for name in (f"jvec{tag}.vti", f"jmod{tag}.vti"):
             out = (tmp_path / name).read_text()
