@@ -146,8 +146,9 @@ void Arena::carve(DevBuf &b, size_t bytes, bool zero, hipStream_t st) {
 }
 
 // Expected cost of a function order: the mean over sample points of
-// (number of live 32-function blocks)^2, the live (K-block, N-block) pairs
-// of a row tile at the point.  live[p * nsh + s]: shell s reaches sample p.
+// (number of live 8-function groups)^2, the live (K-group, N-group) pairs
+// of a row tile at the point, which is what the contraction kernels walk.
+// live[p * nsh + s]: shell s reaches sample p.
 template <class NOut>
 double order_cost(const Basis &b, const std::vector<int> &order, const std::vector<char> &live,
                   int npt, NOut nout_of) {
@@ -159,15 +160,15 @@ double order_cost(const Basis &b, const std::vector<int> &order, const std::vect
         fcnt[si] = nout_of(b.shells[si]);
         pos += fcnt[si];
     }
-    const int nblk = (pos + 31) / 32;
-    std::vector<int> stamp(nblk, -1);
+    const int ngrp = (pos + 7) / 8;
+    std::vector<int> stamp(ngrp, -1);
     double sum = 0.0;
     for (int p = 0; p < npt; p++) {
         const char *lv = live.data() + (size_t)p * nsh;
         int n = 0;
         for (int s = 0; s < nsh; s++) {
             if (!lv[s]) continue;
-            for (int k = fpos[s] >> 5; k <= (fpos[s] + fcnt[s] - 1) >> 5; k++)
+            for (int k = fpos[s] >> 3; k <= (fpos[s] + fcnt[s] - 1) >> 3; k++)
                 if (stamp[k] != p) {
                     stamp[k] = p;
                     n++;
