@@ -154,6 +154,35 @@ def test_each_mixed_kernel_within_tolerance(c4h4, variant, split):
     assert e.max() <= MIXED_TOL
 
 
+def test_mixed_n_tiles_split_any_way_agree_to_fp64_rounding():
+    """the 256-row kernels (8: two LDS buffers, 9: three) with 1, 2, 4 and 8
+    N-ranges per row tile on the 60-atom cage: an N-tile is always the same
+    four live groups of its row tile, so its FP32 sums are the same bits in
+    every configuration and only the FP64 order of adding N-tiles and ranges
+    differs (a tile assigned twice, dropped or built from the wrong groups
+    would differ by far more)"""
+    from gimic_amd import synth
+    s = synth.cage60()
+    ctx = s.context(screening=1e-8)
+    try:
+        ax = np.linspace(-6.0, 6.0, 20)
+        pts = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+        ctx.set_fp64_radius(0.0)
+        out = {}
+        for variant, split in [(9, 4), (9, 1), (9, 2), (9, 8), (8, 4), (8, 1)]:
+            print("case n-tile split", variant, split, flush=True)
+            ctx.set_mixed_kernel(variant, split)
+            out[(variant, split)] = ctx.jtensor(pts, precision="mixed")
+        ref = out[(9, 4)]
+        scale = np.abs(ref).max()
+        for k, J in out.items():
+            d = np.abs(J - ref).max() / scale
+            print(k, "max deviation %.2e" % d)
+            assert d <= 1e-13, (k, d)
+    finally:
+        ctx.close()
+
+
 def test_mixed_open_shell_two_sets(openshell):
     """alpha and beta sets in one mixed launch (pre-split kernel) against
     separate FP64 evaluations"""
