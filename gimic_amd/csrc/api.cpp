@@ -630,10 +630,19 @@ constexpr int kScratchMaxSplit = 32;
 // or 8 x 32)
 int mixed_tile_rows(int variant) { return variant == 5 || variant >= 8 ? 256 : 128; }
 
+// Row tile of the FP64 kernel: 128 rows, or 64 for chunks of up to
+// kSmallF64Points points (the FP64 pass over the points next to nuclei, the
+// nuclei's share of property grids): there neighbouring points are far
+// apart, and the smaller tiles list a fifth fewer group pairs on the
+// 60-atom cage.  The screening lists are sized for the smallest tile.
+constexpr int kMinTileRows = 64;
+constexpr size_t kSmallF64Points = 131072;
+int f64_tile_rows(size_t points) { return points <= kSmallF64Points ? 64 : kWgRows; }
+
 // Arena bytes one Scratch-mode chunk of `rows` rows carves (every carve
 // rounded to 256 bytes), with `split` N-ranges per row tile
 size_t scratch_eval_bytes(const gimic_ctx *c, int prec, int nmatset, size_t rows, int split) {
-    const size_t ld = (size_t)c->lda, nblk = ld / 32, t = rows / 128;
+    const size_t ld = (size_t)c->lda, nblk = ld / 32, t = rows / kMinTileRows;
     auto r = [](size_t b) { return round_up(std::max<size_t>(b, 1), 256); };
     size_t n = prec == GIMIC_PREC_FP64 ? r(rows * ld * 8) + r(3 * rows * ld * 8)
                                        : r(rows * ld * 2 * kAParts) + r(3 * rows * ld * 4);
@@ -773,17 +782,17 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
     const size_t rows_cap = chunk;
     const long long gplane = (long long)(rows_cap * ld);
     // spatial tiles: sorted points, caller indices, sort scratch, and block
-    // lists for row tiles of 128 rows or more
+    // lists for row tiles of kMinTileRows rows or more
     const int nblk = c->lda / 32;
     grow(W.spts, rows_cap * 3 * 8, false);
     grow(W.perm, rows_cap * 4, false);
     grow(W.bpart, 256 * 6 * 8, false);
     grow(W.sortscr, (size_t)sort_scratch_ints((int)rows_cap) * 4, false);
-    grow(W.bcount, rows_cap / 128 * 4, false);
-    grow(W.blist, rows_cap / 128 * nblk * 4, false);
-    grow(W.toff, (rows_cap / 128 + 1) * 4, false);
-    grow(W.k8count, rows_cap / 128 * 4, false);
-    grow(W.k8list, rows_cap / 128 * nblk * 4 * 4, false);
+    grow(W.bcount, rows_cap / kMinTileRows * 4, false);
+    grow(W.blist, rows_cap / kMinTileRows * nblk * 4, false);
+    grow(W.toff, (rows_cap / kMinTileRows + 1) * 4, false);
+    grow(W.k8count, rows_cap / kMinTileRows * 4, false);
+    grow(W.k8list, rows_cap / kMinTileRows * nblk * 4 * 4, false);
     grow(W.bzero, 12 * 1024, W.bzero.bytes < 12 * 1024);  // zeroed once, never written
     const int nmatset = M.nmatset;
     // chunks in rotated order, the middle one first: the autotuner times the
@@ -893,7 +902,7 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         };
         auto plan_variant = [&](int variant, int split = -1) {
             if (split < 0) split = msplit;
-            if (variant == 0) plan(kWgRows, kNT);
+            if (variant == 0) plan(f64_tile_rows(n_c), kNT);
             else plan(mixed_tile_rows(variant), kMx2Cols / kNMat, split);
         };
         // mixed kernels: row-major partial sums (written whole by the launch),
@@ -901,9 +910,9 @@ void evaluate(gimic_ctx *c, int64_t npts, const double *d_r, double *d_jt,
         // and offsets built before the basis evaluation)
         auto run = [&](int variant, int split = -1) {
             plan_variant(variant, split);
-            if (variant == 0) lists(kWgRows);  // the FP64 kernel walks lists over dense rows
+            if (variant == 0) lists(f64_tile_rows(n_c));  // the FP64 kernel walks lists over dense rows
             ca.toff = variant != 0 ? W.toff.as<int>() : nullptr;
-            if (variant == 0) hip_check(launch_contract_f64(ca, st), "contract_f64");
+            if (variant == 0) hip_check(launch_contract_f64(ca, f64_tile_rows(n_c), st), "contract_f64");
             else if (variant == 3) hip_check(launch_contract_mx2(ca, 2, 2, 4, st), "contract_mx2");
             else if (variant == 4) hip_check(launch_contract_mx2(ca, 3, 2, 4, st), "contract_mx2");
             else if (variant == 5) hip_check(launch_contract_mx2(ca, 2, 4, 4, st), "contract_mx2");

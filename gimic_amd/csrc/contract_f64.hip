@@ -15,9 +15,13 @@
 // is accumulated as sum Y0 phi (r - R_nu) (gauge origin at each function's
 // centre), again free of cancellation.
 //
-// Tiling: 256 threads = 4 wavefronts; a workgroup owns 128 point rows
-// (32 per wavefront: two 16-row MFMA blocks) and walks a range of 16-column
-// N-tiles; per N-tile each wavefront keeps 2 x 7 v_mfma_f64_16x16x4 blocks.
+// Tiling: WAVES wavefronts (4, or 2 for small launches); a workgroup owns
+// 32 WAVES point rows (32 per wavefront: two 16-row MFMA blocks) and walks a
+// range of 16-column N-tiles; per N-tile each wavefront keeps 2 x 7
+// v_mfma_f64_16x16x4 blocks.  Row tiles of 64 rather than 128 points list
+// fewer groups where neighbouring points are far apart, as in the FP64 pass
+// over the points next to nuclei, whose 128-row tiles mostly take in two
+// nuclei.
 // Only what screening leaves non-zero over the row tile (tiles.hip) is
 // visited: the live 8-function groups of its listed 32-function blocks,
 // two per 16-column N-tile (split evenly over the workgroups that share
@@ -30,17 +34,19 @@
 
 namespace gimic {
 
-__global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs a) {
-    __shared__ double qs[kWgRows * kNQ];
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void contract_f64_kernel(ContractArgs a) {
+    constexpr int kRows = 32 * WAVES;
+    __shared__ double qs[kRows * kNQ];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, c = lane & 15;
     const int set = blockIdx.z, split = blockIdx.y;
-    const size_t row0 = (size_t)blockIdx.x * kWgRows;
+    const size_t row0 = (size_t)blockIdx.x * kRows;
     const int lda = a.lda;
     const size_t rows = (size_t)a.rows;
     const size_t matsz = (size_t)a.ncol_pad * lda;
     const size_t gplane = (size_t)a.gplane;
-    for (int i = tid; i < kWgRows * kNQ; i += 256) qs[i] = 0.0;
+    for (int i = tid; i < kRows * kNQ; i += 64 * WAVES) qs[i] = 0.0;
     __syncthreads();
 
     const double *A0 = a.a64 + (row0 + wave * 32 + c) * lda;
@@ -139,15 +145,20 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(ContractArgs a) {
     }
     __syncthreads();
     double *out = a.part + ((size_t)set * a.nsplit + split) * kNQ * rows + row0;
-    for (int i = tid; i < kWgRows * kNQ; i += 256) {
-        const int q = i / kWgRows, rl = i % kWgRows;
+    for (int i = tid; i < kRows * kNQ; i += 64 * WAVES) {
+        const int q = i / kRows, rl = i % kRows;
         out[(size_t)q * rows + rl] = qs[rl * kNQ + q];
     }
 }
 
-hipError_t launch_contract_f64(const ContractArgs &a, hipStream_t s) {
-    dim3 grid(a.rows / kWgRows, a.nsplit, a.nmatset);
-    hipLaunchKernelGGL(contract_f64_kernel, grid, dim3(256), 0, s, a);
+hipError_t launch_contract_f64(const ContractArgs &a, int tile_rows, hipStream_t s) {
+    if (tile_rows != 64 && tile_rows != 128) return hipErrorInvalidValue;
+    if (a.rows % tile_rows) return hipErrorInvalidValue;
+    dim3 grid(a.rows / tile_rows, a.nsplit, a.nmatset);
+    if (tile_rows == 64)
+        hipLaunchKernelGGL(contract_f64_kernel<2>, grid, dim3(128), 0, s, a);
+    else
+        hipLaunchKernelGGL(contract_f64_kernel<4>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

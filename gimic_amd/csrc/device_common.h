@@ -135,7 +135,8 @@ struct ContractArgs {
     // compressed operands (pre-split kernels; see EvalArgs): null for dense
     const int *toff;
 };
-hipError_t launch_contract_f64(const ContractArgs &a, hipStream_t s);
+// FP64 kernel: row tiles of tile_rows = 64 or 128 points (a.rows a multiple)
+hipError_t launch_contract_f64(const ContractArgs &a, int tile_rows, hipStream_t s);
 // Pre-split variant: both operands arrive as BF16 parts, the right operand in
 // pair blocks that the kernel gathers into its LDS buffers (see
 // contract_mx2.hip); nbuf 2 or 3 LDS buffers, krb 2 or 4 row blocks per

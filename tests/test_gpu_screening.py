@@ -48,6 +48,25 @@ def test_result_does_not_depend_on_point_order(cage, precision):
     assert np.array_equal(ctx.jtensor(pts, precision=precision), J)
 
 
+def test_fp64_tiles_of_64_and_128_rows_agree(cage):
+    """a chunk of up to 131,072 points takes 64-row FP64 tiles, a larger one
+    128-row tiles: the same points evaluated alone and inside a larger batch
+    give the same tensors to summation-order rounding"""
+    s, ctx = cage
+    pts = _grid(16, 0.2, [4.5, -1.5, -1.5])  # 4096 points next to the cage wall
+    far = np.random.default_rng(8).uniform(-7.0, 7.0, (140000, 3))
+    print("case cage60 fp64 tiles of 64 and 128 rows", flush=True)
+    ctx.set_tile_stats(True)
+    J64 = ctx.jtensor(pts, precision="fp64")
+    assert ctx.tile_stats()["tile_rows"] == 64
+    ctx.set_tile_stats(True)
+    J128 = ctx.jtensor(np.concatenate([pts, far]), precision="fp64")[:len(pts)]
+    assert ctx.tile_stats()["tile_rows"] == 128
+    ctx.set_tile_stats(False)
+    e = rel_err(J64, J128)
+    assert e.max() <= 1e-12, e.max()
+
+
 def test_chunks_change_launches_not_results(cage):
     """a chunk limit that splits the 24^3 points into four mixed launches
     (4096 rows of 1824 padded functions at 18 bytes) and six FP64 launches
@@ -105,7 +124,8 @@ def test_lists_leave_out_blocks_and_fp64_matches_oracle(cage):
     J = ctx.jtensor(pts, precision="fp64")
     st = ctx.tile_stats()
     ctx.set_tile_stats(False)
-    assert st["tile_rows"] == 128 and st["tiles"] == len(pts) // 128
+    # a chunk of up to 131,072 points takes 64-row FP64 tiles
+    assert st["tile_rows"] == 64 and st["tiles"] == len(pts) // 64
     frac = st["blocks"] / (st["tiles"] * st["nblk"])
     assert frac < 0.8, frac  # most of the far side's blocks are screened
     # the live groups are within the listed blocks: k <= 4 n on every tile,
