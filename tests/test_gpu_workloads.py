@@ -13,6 +13,12 @@
   (1e-10 per point); mixed against FP64 over the whole grid (1e-5 per point,
   floor 1e-3 of the largest tensor), both cases from one pass.
 
+* Precision (cage60, C60 truncated icosahedron, nbf 1800, nocc 180) on the
+  128^3 grid at 0.12 bohr: mixed (with its FP64 pass near nuclei) against
+  FP64 over all 2,097,152 points (1e-5 per point), and the flux through the
+  bond plane of atom 1 and its nearest neighbour (140 x 140 order-7 Lobatto
+  points, 14 x 14 bohr) within 1e-6 of the FP64 flux.
+
 Per-point error: |dJ| / max(|J|, 1e-3 max |J|), as everywhere in the tests.
 """
 import numpy as np
@@ -178,3 +184,44 @@ def test_openshell_mixed_full_grid(sheet200):
     # the two cases from one pass equal separate single-case evaluations
     Jt = ctx.jtensor_device(pts[:50000], spin="spindens", precision="fp64")
     assert rel_err(Jt.cpu().numpy(), J64[1, :50000].cpu().numpy()).max() <= 1e-12
+
+
+def test_cage_mixed_full_grid_and_bond_plane_flux():
+    import torch
+    from gimic_amd import flux, synth
+    from gimic_amd import grid as gridmod
+    s = synth.cage60()
+    ctx = s.context(screening=1e-8)
+    try:
+        ctx.set_chunk_bytes(-1)
+        pts = torch.tensor(synth.cube_grid(128, 0.12, -7.62), dtype=torch.float64, device="cuda")
+        assert pts.shape[0] == 128 ** 3
+        print("case cage60 128^3 fp64", flush=True)
+        J64 = ctx.jtensor_device(pts, precision="fp64")
+        print("case cage60 128^3 mixed", flush=True)
+        Jmx = ctx.jtensor_device(pts, precision="mixed")
+        torch.cuda.synchronize()
+        assert ctx.fp64_refined()[1] > 0  # the FP64 pass near nuclei ran
+        nb = torch.linalg.norm(J64, dim=1)
+        d = torch.linalg.norm(Jmx - J64, dim=1)
+        e = (d / torch.clamp(nb, min=1e-3 * float(nb.max()))).cpu().numpy()
+        print("cage60 per-point error max %.2e p99 %.2e" % (e.max(), np.quantile(e, 0.99)))
+        assert e.max() <= 1e-5, e.max()
+        del J64, Jmx
+        c = np.asarray(s.coords)
+        dist = np.linalg.norm(c - c[0], axis=1)
+        dist[0] = np.inf
+        j = int(np.argmin(dist))
+        kw = {"_arg": "bond", "type": "lobatto", "gauss_order": 7, "bond": [1, j + 1],
+              "fixcoord": [0.0, 0.0, 0.0], "distance": float(dist[j] / 2),
+              "height": [-7.0, 7.0], "width": [-7.0, 7.0], "grid_points": [140, 140, 0]}
+        g = gridmod.new_grid(kw, c)
+        B = gridmod.get_magnet(g, "X")
+        print("case cage60 bond plane flux", flush=True)
+        f64 = np.array(flux.integrate_current(ctx, g, B, precision="fp64"))
+        fmx = np.array(flux.integrate_current(ctx, g, B, precision="mixed"))
+        rel = abs(fmx[0] - f64[0]) / abs(f64[0])
+        print("cage60 bond plane flux %.6e, mixed rel error %.2e" % (f64[0], rel))
+        assert abs(f64[0]) > 0.1 and rel <= 1e-6, (f64, fmx)
+    finally:
+        ctx.close()
