@@ -5,12 +5,15 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gimic_hip.h"
@@ -262,6 +265,7 @@ std::vector<int> choose_function_order(const Basis &b, const std::vector<int> &a
 struct gimic_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // host copies of pipelined batches
     size_t chunk_bytes = 0;  // basis buffers of one chunk of points
     size_t chunk_bytes_large = 0;  // the large setting: a third of the device memory
     Basis basis;
@@ -333,6 +337,10 @@ struct gimic_ctx {
         if (stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
+        }
+        if (copy_stream) {
+            (void)hipStreamSynchronize(copy_stream);
+            (void)hipStreamDestroy(copy_stream);
         }
     }
 };
@@ -1099,6 +1107,164 @@ void refine_near_nuclei(gimic_ctx *c, int64_t npts, const double *d_r, double *d
     c->refined_last = k;
 }
 
+// ---- host-array batches: copies beside the evaluation ----
+
+// A host-array batch of more than kPipeMin points (gimic_jtensor_batch)
+// runs in sub-batches: near-equal ones of at most kPipeHead points, then
+// kPipeTail[0] and kPipeTail[1] points.  A helper thread makes the host
+// copies on a second stream of default priority, in order: the points of
+// sub-batch b + 2 go up and the tensors of sub-batch b come down while the
+// caller's thread evaluates sub-batch b + 1 on the context's stream, so the
+// PCIe copies (the host arrays are pageable: each copy holds the thread
+// that issues it until it is done) overlap the kernels.  What stays exposed
+// is the first sub-batch's points going up and the last one's tensors
+// coming down, hence the short tail: the tensors of a sub-batch come down
+// (about 7 ns a point into fresh pageable memory on MI355X's host link)
+// while the next, a quarter its size, is evaluated (about 33 ns a point on
+// the 60-atom cage), and each sub-batch costs about 1 ms of its own.  Each
+// sub-batch has its own events: "up" (its points are on the device) is
+// waited for by the context's stream, "done" (its tensors are written) by
+// the copy stream.  A sub-batch is evaluated like a call of its own, so
+// mixed results agree with an unsplit evaluation to the mixed tolerance and
+// FP64 results to summation-order rounding.
+constexpr int64_t kPipeHead = 1048576;
+constexpr int64_t kPipeTail[2] = {262144, 65536};
+constexpr int64_t kPipeMin = kPipeHead + kPipeTail[0] + kPipeTail[1];
+
+void pipelined_batch(gimic_ctx *c, int64_t npts, const double *r, double *jt, int spin,
+                     int precision) {
+    hipStream_t st = c->stream;
+    std::vector<int64_t> offs{0};  // sub-batch b: points offs[b] .. offs[b + 1]
+    const int64_t head = npts - kPipeTail[0] - kPipeTail[1];
+    const int64_t nhead = (head + kPipeHead - 1) / kPipeHead;
+    for (int64_t k = 1; k <= nhead; k++) offs.push_back(head * k / nhead);
+    offs.push_back(head + kPipeTail[0]);
+    offs.push_back(npts);
+    const int nb = (int)offs.size() - 1;
+    auto first = [&](int b) { return offs[b]; };
+    auto count = [&](int b) { return offs[b + 1] - offs[b]; };
+    double *d_r = c->pts_tmp.as<double>(), *d_jt = c->jt_tmp.as<double>();
+    if (!c->copy_stream) hip_check(hipStreamCreate(&c->copy_stream), "hipStreamCreate");
+    hipStream_t cs = c->copy_stream;
+    std::vector<hipEvent_t> up(nb, nullptr), done(nb, nullptr);
+    struct Events {
+        std::vector<hipEvent_t> *v[2];
+        ~Events() {
+            for (auto *w : v)
+                for (hipEvent_t e : *w)
+                    if (e) (void)hipEventDestroy(e);
+        }
+    } events{{&up, &done}};
+    for (int b = 0; b < nb; b++) {
+        hip_check(hipEventCreateWithFlags(&up[b], hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&done[b], hipEventDisableTiming), "hipEventCreate");
+    }
+    // copy jobs in the order the helper runs them: +b + 1 = points of b up,
+    // -(b + 1) = tensors of b down (after its "done" event is recorded)
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<int> jobs;
+    size_t next_job = 0;
+    int ups = 0;          // sub-batches whose points are up and "up" recorded
+    int dones = 0;        // sub-batches whose "done" the main thread recorded
+    int downs = 0;        // sub-batches whose tensors are down
+    bool stop = false;
+    std::exception_ptr err;
+    const int dev = c->device;
+    std::thread helper([&] {
+        try {
+            hip_check(hipSetDevice(dev), "hipSetDevice");
+            for (;;) {
+                int job = 0;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&] { return stop || next_job < jobs.size(); });
+                    if (next_job == jobs.size()) return;
+                    job = jobs[next_job++];
+                }
+                if (job > 0) {
+                    const int b = job - 1;
+                    hip_check(hipMemcpyAsync(d_r + 3 * first(b), r + 3 * first(b),
+                                             (size_t)count(b) * 24, hipMemcpyHostToDevice, cs),
+                              "hipMemcpyAsync");
+                    hip_check(hipEventRecord(up[b], cs), "hipEventRecord");
+                    std::lock_guard<std::mutex> lk(mu);
+                    ups = b + 1;
+                } else {
+                    const int b = -job - 1;
+                    {
+                        std::unique_lock<std::mutex> lk(mu);
+                        cv.wait(lk, [&] { return stop || dones > b; });
+                        if (dones <= b) return;
+                    }
+                    hip_check(hipStreamWaitEvent(cs, done[b], 0), "hipStreamWaitEvent");
+                    hip_check(hipMemcpyAsync(jt + 9 * first(b), d_jt + 9 * first(b),
+                                             (size_t)count(b) * 72, hipMemcpyDeviceToHost, cs),
+                              "hipMemcpyAsync");
+                    hip_check(hipStreamSynchronize(cs), "hipStreamSynchronize");
+                    std::lock_guard<std::mutex> lk(mu);
+                    downs = b + 1;
+                }
+                cv.notify_all();
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(mu);
+            err = std::current_exception();
+            stop = true;
+            cv.notify_all();
+        }
+    });
+    // ends the helper (on an error too) before the frame's state goes away
+    struct Join {
+        std::thread &t;
+        std::mutex &m;
+        std::condition_variable &v;
+        bool &stop;
+        ~Join() {
+            {
+                std::lock_guard<std::mutex> lk(m);
+                stop = true;
+            }
+            v.notify_all();
+            if (t.joinable()) t.join();
+        }
+    } join{helper, mu, cv, stop};
+    auto push = [&](int job) {
+        std::lock_guard<std::mutex> lk(mu);
+        jobs.push_back(job);
+        cv.notify_all();
+    };
+    push(1);
+    push(2);
+    for (int b = 0; b < nb; b++) {
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return err || ups > b; });
+            if (err) break;
+        }
+        hip_check(hipStreamWaitEvent(st, up[b], 0), "hipStreamWaitEvent");
+        evaluate(c, count(b), d_r + 3 * first(b), d_jt + 9 * first(b), {spin}, precision, st,
+                 nullptr);
+        hip_check(hipEventRecord(done[b], st), "hipEventRecord");
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            dones = b + 1;
+        }
+        push(-(b + 1));
+        if (b + 2 < nb) push(b + 3);
+    }
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return err || downs == nb; });
+    }
+    if (err) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamSynchronize(cs);
+        std::rethrow_exception(err);
+    }
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
 // ---- allocation-free evaluation (gimic_jtensor_batch_device_noalloc) ----
 
 // Work memory of the allocation-free entry point: one copy on every GPU the
@@ -1368,6 +1534,10 @@ int gimic_jtensor_batch(gimic_ctx *c, int64_t npts, const double *r, double *jt,
         hipStream_t st = c->stream;
         c->pts_tmp.alloc((size_t)npts * 3 * 8, st, false);
         c->jt_tmp.alloc((size_t)npts * 9 * 8, st, false);
+        if (npts > kPipeMin) {
+            pipelined_batch(c, npts, r, jt, spin, precision);
+            return;
+        }
         hip_check(hipMemcpyAsync(c->pts_tmp.p, r, (size_t)npts * 24, hipMemcpyHostToDevice, st),
                   "hipMemcpyAsync");
         evaluate(c, npts, c->pts_tmp.as<double>(), c->jt_tmp.as<double>(), {spin}, precision, st,

@@ -208,7 +208,12 @@ int gimic_ctx_graph_stats(const gimic_ctx *ctx, long long *builds, long long *la
 int gimic_ctx_destroy(gimic_ctx *ctx);
 
 /* J tensors at npts points: r[3*i..] (bohr) -> jt[9*i + m + 3*b].
- * Host pointers; the call returns when the results are in jt. */
+ * Host pointers; the call returns when the results are in jt.  More than
+ * 1,376,256 points are evaluated in sub-batches whose host copies overlap
+ * the kernels of their neighbours (a helper thread and a second stream of
+ * the context); each sub-batch is evaluated as a call of its own, so mixed
+ * results agree with a one-call device evaluation to the mixed tolerance
+ * and FP64 results to summation-order rounding. */
 int gimic_jtensor_batch(gimic_ctx *ctx, int64_t npts, const double *r, double *jt, int spin,
                         int precision);
 

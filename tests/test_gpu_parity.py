@@ -113,6 +113,26 @@ def test_ragged_batch_sizes(ring12, npts):
         assert rel_err(J, ref).max() <= FP64_TOL
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_host_batch_in_sub_batches_matches_device_path(ring12, precision):
+    """a host-array batch of more than 1,376,256 points runs in sub-batches
+    (here 2 x 619,233, 262,144 and 65,536 points) whose copies overlap the
+    evaluation of their neighbours: every point equals the one-launch device
+    evaluation (FP64 to summation-order rounding, mixed to its tolerance)"""
+    import torch
+    s, ctx = ring12
+    pts = np.random.default_rng(11).uniform(-6.0, 6.0, (1_566_146, 3))
+    pts[::97] *= 0.05  # points next to the nuclei too (the FP64 pass)
+    print("case ring12 host batch 1,566,146 points", precision, flush=True)
+    J = ctx.jtensor(pts, precision=precision)
+    d = torch.tensor(pts, dtype=torch.float64, device="cuda")
+    Jd = ctx.jtensor_device(d, precision=precision).cpu().numpy()
+    e = rel_err(J, Jd)
+    assert e.max() <= (1e-12 if precision == "fp64" else 1e-5), e.max()
+    # and repeats bit for bit
+    assert np.array_equal(ctx.jtensor(pts, precision=precision), J)
+
+
 def test_mixed_precision_within_stated_tolerance(ring12, c4h4):
     s, ctx = ring12
     g = golden("ring12_ref.npz")
