@@ -236,6 +236,23 @@ def test_beta_on_closed_shell_is_an_error(ring12):
         ctx.jtensor(np.zeros((4, 3)), spin="beta")
 
 
+def test_error_inside_a_host_batch_in_sub_batches(ring12):
+    """an error raised by the first sub-batch's evaluation (beta on a closed
+    shell, found on the host) ends the copy thread and reaches the caller;
+    the context then evaluates large host batches as before"""
+    from gimic_amd.lib import GimicError
+    s, ctx = ring12
+    pts = np.random.default_rng(5).uniform(-6.0, 6.0, (1_400_000, 3))
+    print("case ring12 host batch 1,400,000 points, beta on a closed shell", flush=True)
+    with pytest.raises(GimicError, match="not open-shell"):
+        ctx.jtensor(pts, spin="beta")
+    J = ctx.jtensor(pts)
+    g = golden("ring12_ref.npz")
+    ref = ctx.jtensor(g["points"])
+    assert rel_err(ref, g["jt_111"]).max() <= FP64_TOL
+    assert np.isfinite(J).all() and np.abs(J).max() > 0
+
+
 def test_drop_in_c_abi(c4h4_files):
     """gimic_init / set_magnet / set_spin / calc_jtensor / calc_jvector as the
     reference's C interface (gimic_interface.f90:26-151)"""
