@@ -1236,6 +1236,7 @@ void pipelined_batch(gimic_ctx *c, int64_t npts, const double *r, double *jt, in
     };
     push(1);
     push(2);
+    long long refined = 0;  // points of the FP64 pass over all sub-batches
     for (int b = 0; b < nb; b++) {
         {
             std::unique_lock<std::mutex> lk(mu);
@@ -1245,6 +1246,7 @@ void pipelined_batch(gimic_ctx *c, int64_t npts, const double *r, double *jt, in
         hip_check(hipStreamWaitEvent(st, up[b], 0), "hipStreamWaitEvent");
         evaluate(c, count(b), d_r + 3 * first(b), d_jt + 9 * first(b), {spin}, precision, st,
                  nullptr);
+        refined += c->refined_last;
         hip_check(hipEventRecord(done[b], st), "hipEventRecord");
         {
             std::lock_guard<std::mutex> lk(mu);
@@ -1263,6 +1265,7 @@ void pipelined_batch(gimic_ctx *c, int64_t npts, const double *r, double *jt, in
         std::rethrow_exception(err);
     }
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (precision == GIMIC_PREC_MIXED) c->refined_last = refined;
 }
 
 // ---- allocation-free evaluation (gimic_jtensor_batch_device_noalloc) ----

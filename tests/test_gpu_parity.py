@@ -129,6 +129,11 @@ def test_host_batch_in_sub_batches_matches_device_path(ring12, precision):
     Jd = ctx.jtensor_device(d, precision=precision).cpu().numpy()
     e = rel_err(J, Jd)
     assert e.max() <= (1e-12 if precision == "fp64" else 1e-5), e.max()
+    if precision == "mixed":
+        # the FP64 pass near nuclei counts the points of every sub-batch
+        n_dev = ctx.fp64_refined()[1]
+        ctx.jtensor(pts, precision=precision)
+        assert ctx.fp64_refined()[1] == n_dev > 0
     # and repeats bit for bit
     assert np.array_equal(ctx.jtensor(pts, precision=precision), J)
 
